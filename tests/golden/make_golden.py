@@ -1,0 +1,86 @@
+"""Writes the golden fixtures of tests/golden/*.npz from the reference build.
+
+Run from the repository root in the container that holds /root/reference,
+after `make -C oracle` has built oracle/_ref/libthcm_ref.so:
+
+    python tests/golden/make_golden.py
+
+Each fixture holds the inputs of one evaluation point (configuration, land
+mask, surface forcing fields, continuation parameters, state, integral
+correction) and what the reference THCM Fortran -- driven by the one-rank
+restatement of the THCM.C glue in oracle/ref_harness.c -- returns for them:
+F(x) as Ocean::computeRHS gives it, the Jacobian on the fixed graph (row
+pointer, columns, values; all rows, or a seeded row sample for the larger
+grids), diag(B), the integral-condition coefficients and row.  Land masks of
+the natl8 / gateway / global cases are the reference's own data files
+(data/mkmask), read by the reference's readmask.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from thcm_cases import (Case, RefModel, analytic_forcing, box16_case, config_dict,  # noqa: E402
+                        make_config, rand_state)
+
+
+def row_sample(ndim, frac, seed):
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(ndim, size=max(1, int(frac * ndim)), replace=False))
+    return rows.astype(np.int32)
+
+
+def write(case, row_frac=None):
+    ref = RefModel(case)
+    F = ref.rhs()
+    rp, ci, val, dB, dropped = ref.jacobian()
+    assert dropped == 0, "reference produced entries outside the fixed graph"
+    out = dict(cfg=json.dumps(config_dict(case.cfg)), pars=json.dumps(case.pars),
+               landm=case.landm, x=case.x, intcorr=np.float64(case.intcorr),
+               F=F, diagB=dB, coef=ref.coef(), rowintcon=np.int32(ref.rowintcon()))
+    for k, v in case.fields.items():
+        if v is not None:
+            out["fld_" + k] = v
+    if row_frac is None:
+        out.update(J_rowptr=rp, J_col=ci, J_val=val)
+    else:
+        rows = row_sample(case.ndim, row_frac, 17)
+        if ref.rowintcon() >= 0 and ref.rowintcon() not in rows:
+            rows = np.sort(np.append(rows, np.int32(ref.rowintcon())))
+        lens = rp[rows + 1] - rp[rows]
+        sub_rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        idx = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in rows])
+        out.update(J_rows=rows, J_rowptr=sub_rp, J_col=ci[idx], J_val=val[idx])
+    path = os.path.join(HERE, case.name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{path}: ndim={case.ndim} |F|max={np.abs(F).max():.3e} "
+          f"|J|max={np.abs(val).max():.3e} size={os.path.getsize(path)} B")
+
+
+def cases():
+    yield box16_case(), None
+    cfg = make_config(n=8, m=8, l=4, rho_mixing=0, sres=0)
+    landm = RefModel.reference_landm(cfg, 0, 1, "mask_natl8")
+    yield Case("natl8", cfg, landm, {}, dict(COMB=0.7, WIND=1.0, SALT=1.0, TEMP=10.0),
+               rand_state(cfg, 3)), None
+    cfg = make_config(n=8, m=8, l=4, rho_mixing=1, sres=1)
+    yield Case("natl8_eos", cfg, landm, {}, dict(COMB=1.0, WIND=1.0, SALT=1.0, TEMP=10.0,
+                                                  NLES=1.0), rand_state(cfg, 4, 2.0)), None
+    cfg = make_config(n=16, m=16, l=16, periodic=1, xmin_deg=0, xmax_deg=360, ymin_deg=-80,
+                      ymax_deg=80, hdim=5000, qz=2.25, sres=0)
+    landm = RefModel.reference_landm(cfg, 0, 1, "mask_gateway")
+    yield Case("gateway16", cfg, landm, {}, dict(COMB=1.0, WIND=1.0, SALT=1.0, TEMP=10.0),
+               rand_state(cfg, 5)), None
+    cfg = make_config(n=96, m=38, l=12, periodic=1, xmin_deg=0, xmax_deg=360, ymin_deg=-81,
+                      ymax_deg=81, hdim=5000, qz=2.25, sres=0, ih=1, iza=3, ite=2, its=2)
+    landm = RefModel.reference_landm(cfg, 0, 1, "mask_global_96x38x12")
+    yield Case("global96", cfg, landm, analytic_forcing(cfg),
+               dict(COMB=1.0, WIND=1.0, SALT=1.0, TEMP=1.0), rand_state(cfg, 6)), 0.02
+
+
+if __name__ == "__main__":
+    for case, frac in cases():
+        write(case, frac)
