@@ -1,0 +1,379 @@
+// extern "C" layer of include/iemic_amd.h: argument checks, host/device
+// staging, and the Ocean-facade entry points.  The numerical work is in
+// host_setup.cpp (tables), assembly.hip (F, J, products), precond.hip and
+// solver.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "ctx.hpp"
+
+using namespace iemic;
+
+namespace iemic {
+const char *last_error();
+int precond_build(iemic_ctx *c);
+int precond_apply(iemic_ctx *c, const double *d_in, double *d_out);
+void precond_free(iemic_ctx *c);
+int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
+                 iemic_solve_info *info);
+void krylov_free(iemic_ctx *c);
+}  // namespace iemic
+
+#define CHECK_CTX(c)                                                            \
+    do {                                                                        \
+        if (!(c)) return set_error(IEMIC_EINVAL, "null context");               \
+    } while (0)
+
+static int ensure_staging(iemic_ctx *c)
+{
+    if (!c->d_x) {
+        HIPCHK(hipMalloc(&c->d_x, (size_t)c->sc.ndim * sizeof(double)));
+        HIPCHK(hipMalloc(&c->d_y, (size_t)c->sc.ndim * sizeof(double)));
+    }
+    return IEMIC_OK;
+}
+
+extern "C" {
+
+const char *iemic_last_error(void) { return iemic::last_error(); }
+
+int iemic_device_count(int *count)
+{
+    if (!count) return set_error(IEMIC_EINVAL, "null pointer");
+    HIPCHK(hipGetDeviceCount(count));
+    return IEMIC_OK;
+}
+
+int iemic_create(const iemic_config *cfg, const int *landm, const double *taux, const double *tauy,
+                 const double *tatm, const double *emip, const double *spert, iemic_ctx **out)
+{
+    if (!cfg || !landm || !out) return set_error(IEMIC_EINVAL, "null argument");
+    *out = nullptr;
+    iemic_ctx *c = new iemic_ctx();
+    c->cfg = *cfg;
+    int rc = host_init(c, landm, taux, tauy, tatm, emip, spert);
+    if (rc != IEMIC_OK) {
+        delete c;
+        return rc;
+    }
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        delete c;
+        return set_error(IEMIC_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    rc = upload_tables(c);
+    if (rc != IEMIC_OK) {
+        iemic_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return IEMIC_OK;
+}
+
+int iemic_destroy(iemic_ctx *c)
+{
+    if (!c) return IEMIC_OK;
+    precond_free(c);
+    krylov_free(c);
+    void *bufs[] = {c->d_landm, c->d_al, c->d_jt, c->d_kt, c->d_frc, c->d_coef, c->d_diagB,
+                    c->d_J, c->d_x, c->d_y, c->d_red, c->d_src, c->d_col_export};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto &ev : c->ev)
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return IEMIC_OK;
+}
+
+int iemic_set_par(iemic_ctx *c, int idx, double value)
+{
+    CHECK_CTX(c);
+    if (idx < 1 || idx > 30) return set_error(IEMIC_EINVAL, "parameter index must be 1..30");
+    c->h.par[idx] = value;
+    if (idx == 6 || idx == 29) {   // MIXP, MKAP: neutral physics / GM
+        if (value != 0.0) return set_error(IEMIC_EUNSUPPORTED, "neutral physics (MIXP/MKAP) is not supported");
+    }
+    host_parameters_changed(c);
+    return upload_tables(c);
+}
+
+int iemic_get_par(const iemic_ctx *c, int idx, double *value)
+{
+    CHECK_CTX(c);
+    if (idx < 1 || idx > 30 || !value) return set_error(IEMIC_EINVAL, "parameter index must be 1..30");
+    *value = c->h.par[idx];
+    return IEMIC_OK;
+}
+
+int iemic_ndim(const iemic_ctx *c, int *ndim)
+{
+    CHECK_CTX(c);
+    *ndim = c->sc.ndim;
+    return IEMIC_OK;
+}
+
+int iemic_graph_nnz(const iemic_ctx *c, int *nnz)
+{
+    CHECK_CTX(c);
+    if (!c->graph_built) build_graph(const_cast<iemic_ctx *>(c)), const_cast<iemic_ctx *>(c)->graph_built = true;
+    *nnz = c->graph.nnz;
+    return IEMIC_OK;
+}
+
+int iemic_rowintcon(const iemic_ctx *c, int *row)
+{
+    CHECK_CTX(c);
+    *row = c->sc.rowintcon;
+    return IEMIC_OK;
+}
+
+int iemic_intcond_coef(const iemic_ctx *c, double *coef)
+{
+    CHECK_CTX(c);
+    std::memcpy(coef, c->h.coef.data(), sizeof(double) * c->sc.ndim);
+    return IEMIC_OK;
+}
+
+int iemic_forcing(const iemic_ctx *c, double *frc)
+{
+    CHECK_CTX(c);
+    std::memcpy(frc, c->h.frc.data(), sizeof(double) * c->sc.ndim);
+    return IEMIC_OK;
+}
+
+int iemic_set_intcond_correction(iemic_ctx *c, const double *x)
+{
+    CHECK_CTX(c);
+    double d = 0.0;
+    for (int r = 0; r < c->sc.ndim; r++) d += c->h.coef[r] * x[r];
+    c->intcorr = d;
+    return IEMIC_OK;
+}
+
+int iemic_set_intcond_correction_value(iemic_ctx *c, double value)
+{
+    CHECK_CTX(c);
+    c->intcorr = value;
+    return IEMIC_OK;
+}
+
+int iemic_rhs_dev(iemic_ctx *c, const double *d_x, double *d_F)
+{
+    CHECK_CTX(c);
+    return launch_rhs(c, d_x, d_F);
+}
+
+int iemic_rhs(iemic_ctx *c, const double *x, double *F)
+{
+    CHECK_CTX(c);
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    size_t bytes = (size_t)c->sc.ndim * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_x, x, bytes, hipMemcpyHostToDevice, c->stream));
+    rc = launch_rhs(c, c->d_x, c->d_y);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(F, c->d_y, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IEMIC_OK;
+}
+
+int iemic_jacobian_dev(iemic_ctx *c, const double *d_x)
+{
+    CHECK_CTX(c);
+    return launch_jacobian(c, d_x);
+}
+
+int iemic_jacobian(iemic_ctx *c, const double *x)
+{
+    CHECK_CTX(c);
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_x, x, (size_t)c->sc.ndim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    rc = launch_jacobian(c, c->d_x);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IEMIC_OK;
+}
+
+int iemic_jacobian_csr(iemic_ctx *c, int *rowptr, int *col, double *val, double *diagB)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    if (!c->graph_built) {
+        build_graph(c);
+        c->graph_built = true;
+    }
+    Graph &g = c->graph;
+    if (!c->d_src) {
+        HIPCHK(hipMalloc(&c->d_src, (size_t)g.nnz * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_col_export, (size_t)g.nnz * sizeof(int)));
+        HIPCHK(hipMemcpy(c->d_src, g.src.data(), (size_t)g.nnz * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_col_export, g.col.data(), (size_t)g.nnz * sizeof(int), hipMemcpyHostToDevice));
+    }
+    double *d_val = nullptr;
+    HIPCHK(hipMalloc(&d_val, (size_t)g.nnz * sizeof(double)));
+    int rc = launch_export(c, d_val);
+    if (rc == IEMIC_OK) {
+        hipError_t e = hipMemcpyAsync(val, d_val, (size_t)g.nnz * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = set_error(IEMIC_EHIP, hipGetErrorString(e));
+    }
+    (void)hipFree(d_val);
+    if (rc) return rc;
+    if (rowptr) std::memcpy(rowptr, g.rowptr.data(), sizeof(int) * (c->sc.ndim + 1));
+    if (col) std::memcpy(col, g.col.data(), sizeof(int) * g.nnz);
+    if (diagB) std::memcpy(diagB, c->h.diagB.data(), sizeof(double) * c->sc.ndim);
+    return IEMIC_OK;
+}
+
+int iemic_apply_matrix_dev(iemic_ctx *c, const double *d_v, double *d_out)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    return launch_spmv(c, d_v, d_out);
+}
+
+int iemic_apply_matrix_transpose_dev(iemic_ctx *c, const double *d_v, double *d_out)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    return launch_spmv_t(c, d_v, d_out);
+}
+
+int iemic_apply_matrix(iemic_ctx *c, const double *v, double *out)
+{
+    CHECK_CTX(c);
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    size_t bytes = (size_t)c->sc.ndim * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_x, v, bytes, hipMemcpyHostToDevice, c->stream));
+    rc = iemic_apply_matrix_dev(c, c->d_x, c->d_y);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, c->d_y, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IEMIC_OK;
+}
+
+int iemic_apply_mass(iemic_ctx *c, const double *v, double *out)
+{
+    CHECK_CTX(c);
+    for (int r = 0; r < c->sc.ndim; r++) out[r] = c->h.diagB[r] * v[r];
+    return IEMIC_OK;
+}
+
+int iemic_build_preconditioner(iemic_ctx *c)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    return precond_build(c);
+}
+
+int iemic_apply_precon_dev(iemic_ctx *c, const double *d_v, double *d_out)
+{
+    CHECK_CTX(c);
+    return precond_apply(c, d_v, d_out);
+}
+
+int iemic_apply_precon(iemic_ctx *c, const double *v, double *out)
+{
+    CHECK_CTX(c);
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    size_t bytes = (size_t)c->sc.ndim * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_x, v, bytes, hipMemcpyHostToDevice, c->stream));
+    rc = precond_apply(c, c->d_x, c->d_y);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, c->d_y, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IEMIC_OK;
+}
+
+void iemic_solver_defaults(iemic_solver_params *p)
+{
+    if (!p) return;
+    p->tol = 1e-8;
+    p->restart = 250;
+    p->maxit = 1000;
+    p->precision = 0;
+    p->inner_restart = 100;
+    p->inner_tol = 1e-3;
+    p->verbose = 0;
+}
+
+int iemic_solve_dev(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
+                    iemic_solve_info *info)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return krylov_solve(c, p ? p : &def, d_b, d_x, info);
+}
+
+int iemic_solve(iemic_ctx *c, const iemic_solver_params *p, const double *b, double *x,
+                iemic_solve_info *info)
+{
+    CHECK_CTX(c);
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    size_t bytes = (size_t)c->sc.ndim * sizeof(double);
+    double *db = nullptr, *dx = nullptr;
+    HIPCHK(hipMalloc(&db, bytes));
+    HIPCHK(hipMalloc(&dx, bytes));
+    rc = IEMIC_OK;
+    if (hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy b");
+    if (!rc) rc = iemic_solve_dev(c, p, db, dx, info);
+    if (rc == IEMIC_OK || rc == IEMIC_ENOCONV) {
+        if (hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy x");
+    }
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    return rc;
+}
+
+int iemic_dev_alloc(void **ptr, long long bytes)
+{
+    if (!ptr || bytes < 0) return set_error(IEMIC_EINVAL, "bad allocation request");
+    hipError_t e = hipMalloc(ptr, (size_t)bytes);
+    if (e != hipSuccess) return set_error(IEMIC_ENOMEM, hipGetErrorString(e));
+    return IEMIC_OK;
+}
+int iemic_dev_free(void *ptr)
+{
+    if (ptr) HIPCHK(hipFree(ptr));
+    return IEMIC_OK;
+}
+int iemic_memcpy_h2d(void *dst, const void *src, long long bytes)
+{
+    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return IEMIC_OK;
+}
+int iemic_memcpy_d2h(void *dst, const void *src, long long bytes)
+{
+    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return IEMIC_OK;
+}
+int iemic_synchronize(iemic_ctx *c)
+{
+    CHECK_CTX(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IEMIC_OK;
+}
+
+int iemic_last_kernel_ms(const iemic_ctx *c, int kernel, double *ms)
+{
+    CHECK_CTX(c);
+    if (kernel < 0 || kernel >= 8 || !ms) return set_error(IEMIC_EINVAL, "bad kernel id");
+    if (!c->ev[kernel][0]) return set_error(IEMIC_ESTATE, "kernel not launched yet");
+    float t = 0.f;
+    HIPCHK(hipEventSynchronize(c->ev[kernel][1]));
+    HIPCHK(hipEventElapsedTime(&t, c->ev[kernel][0], c->ev[kernel][1]));
+    *ms = t;
+    return IEMIC_OK;
+}
+
+}  // extern "C"

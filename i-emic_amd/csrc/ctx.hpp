@@ -1,0 +1,93 @@
+// Context of one ocean model: host-side set-up data and device buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/iemic_amd.h"
+#include "model.hpp"
+
+namespace iemic {
+
+struct Precond;   // preconditioner (precond.hip)
+struct Krylov;    // solver workspace (solver.hip)
+
+struct Host {
+    // grid (grid.F90), Fortran lower bounds kept: x[1..n], y[0..m+1],
+    // yv[0..m], z/dfzT[1..l], dfzW[0..l]
+    std::vector<double> x, y, xu, yv, z, zw, dfzT, dfzW;
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0, dx = 0, dy = 0, dz = 0;
+    std::vector<int> landm;          // (n+2)(m+2)(l+2), after init_ border rules
+    double par[31] = {0};
+    std::vector<double> taux, tauy, tatm, emip, spert;  // n*m
+    std::vector<double> frc;         // ndim, after boundary zeroing
+    std::vector<double> coef;        // ndim, integral-condition coefficients
+    std::vector<double> diagB;       // ndim, THCM::evaluate diag(B)
+    std::vector<double> al;          // m*l*104 linear slot values per (j,k)
+    std::vector<double> jt, kt;      // NJT*(m+2), NKT*(l+2)
+};
+
+struct Graph {  // reference CSR of the fixed graph and its slot map
+    std::vector<int> rowptr, col;    // ndim+1, nnz
+    std::vector<int> src;            // nnz: slot-format index (s*ncell+c), -1 for
+                                     // the integral row entries
+    int nnz = 0;
+};
+
+}  // namespace iemic
+
+struct iemic_ctx {
+    iemic_config cfg{};
+    iemic::Scalars sc{};
+    iemic::Host h;
+    iemic::Graph graph;
+    bool graph_built = false;
+    double intcorr = 0.0;
+    int vmix_flag = 1;
+
+    hipStream_t stream = nullptr;
+    // device data
+    uint8_t *d_landm = nullptr;
+    double *d_al = nullptr, *d_jt = nullptr, *d_kt = nullptr;
+    double *d_frc = nullptr, *d_coef = nullptr, *d_diagB = nullptr;
+    double *d_J = nullptr;           // 104*ncell slot values
+    double *d_x = nullptr, *d_y = nullptr;  // staging for host-pointer entry points
+    double *d_red = nullptr;         // reduction partials
+    int nred = 0;
+    int *d_src = nullptr;            // CSR export map (slot index per entry)
+    int *d_col_export = nullptr;     // CSR export columns
+    bool have_jac = false;
+    iemic::Precond *prec = nullptr;
+    iemic::Krylov *kry = nullptr;
+    // timing
+    hipEvent_t ev[8][2] = {};
+    double last_ms[8] = {0};
+};
+
+namespace iemic {
+// host_setup.cpp
+int host_init(iemic_ctx *c, const int *landm, const double *taux, const double *tauy,
+              const double *tatm, const double *emip, const double *spert);
+void host_parameters_changed(iemic_ctx *c);   // forcing + linear tables (setparcs_)
+void build_graph(iemic_ctx *c);
+int set_error(int code, const std::string &msg);
+// assembly.hip
+int upload_tables(iemic_ctx *c);
+int launch_rhs(iemic_ctx *c, const double *d_x, double *d_F);
+int launch_jacobian(iemic_ctx *c, const double *d_x);
+int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y);
+int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y);
+int launch_export(iemic_ctx *c, double *d_val);
+double device_dot(iemic_ctx *c, const double *d_a, const double *d_b, int n);
+void timed_begin_ext(iemic_ctx *c, int which);
+void timed_end_ext(iemic_ctx *c, int which);
+}  // namespace iemic
+
+#define HIPCHK(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return iemic::set_error(IEMIC_EHIP, std::string(#expr) + ": " +       \
+                                                    hipGetErrorString(e_));       \
+    } while (0)

@@ -1,0 +1,452 @@
+// Right-preconditioned flexible GMRES on the device (the solve of
+// Ocean::solve, Ocean.C:1070-1147, which runs Belos FGMRES with DGKS
+// orthogonalisation; the in-tree template GMRESSolver.H:80-255 has the same
+// structure).  Krylov vectors live in HBM; each iteration is
+//   z_j = M^{-1} v_j,  w = J z_j,
+//   two passes of classical Gram-Schmidt (h = V^T w, w -= V h),
+//   h_{j+1,j} = ||w||, v_{j+1} = w / h_{j+1,j},
+// and the small Hessenberg least-squares problem is solved on the host with
+// Givens rotations (GMRESSolver.H:257-289).  The FP64 true residual
+// ||b - J x|| / ||b|| is checked at every restart and at the end.
+//
+// Mixed precision (precision = 1): an FP64 outer iterative refinement
+// r = b - J x (FP64 SpMV on the FP64 Jacobian), inner solve of J d = r with
+// FGMRES whose basis, SpMV (FP32 copy of J) and preconditioner run in FP32,
+// x += d, until the FP64 relative residual reaches tol.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace iemic {
+
+int precond_build(iemic_ctx *c);
+int precond_apply(iemic_ctx *c, const double *d_in, double *d_out);
+int precond_apply_f32(iemic_ctx *c, const float *d_in, float *d_out);
+int precond_build_f32(iemic_ctx *c);
+
+template <typename T>
+struct Basis {
+    T *V = nullptr, *Z = nullptr, *w = nullptr;
+    int cap = 0;
+};
+
+struct Krylov {
+    Basis<double> b64;
+    Basis<float> b32;
+    double *d_h = nullptr;     // (restart+1) coefficients
+    double *d_part = nullptr;  // partial sums (blocks x (restart+1))
+    int nblocks = 0;
+    double *d_r = nullptr, *d_d = nullptr;   // FP64 refinement residual / correction
+    float *J32 = nullptr;                    // FP32 copy of the slot values
+    float *coef32 = nullptr;
+    int hcap = 0;
+};
+
+// ------------------------------------------------------------------ kernels
+template <typename T>
+__global__ void __launch_bounds__(256) k_multidot(const T *__restrict__ V, size_t ld, int nv,
+                                                   const T *__restrict__ w, int n,
+                                                   double *__restrict__ part)
+{
+    // part[q * gridDim.x + block] = sum over this block's rows of V[q] . w
+    __shared__ double sh[256];
+    int chunk = (n + gridDim.x - 1) / gridDim.x;
+    int r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+    for (int q = 0; q < nv; q++) {
+        const T *v = V + (size_t)q * ld;
+        double acc = 0.0;
+        for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) acc += (double)v[r] * (double)w[r];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_sum_parts(const double *__restrict__ part, int nblocks, int nv,
+                            double *__restrict__ out, int accumulate)
+{
+    // one block per vector: out[q] (+)= sum_b part[q*nblocks + b]
+    __shared__ double sh[256];
+    int q = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += part[(size_t)q * nblocks + b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = accumulate ? out[q] + sh[0] : sh[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_update(T *__restrict__ w, const T *__restrict__ V, size_t ld,
+                                                 int nv, const double *__restrict__ h, double sign, int n)
+{
+    // w += sign * V h
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double acc = 0.0;
+    for (int q = 0; q < nv; q++) acc += (double)V[(size_t)q * ld + r] * h[q];
+    w[r] = (T)((double)w[r] + sign * acc);
+}
+
+template <typename T>
+__global__ void k_scale_copy(T *__restrict__ dst, const T *__restrict__ src, double a, int n)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) dst[r] = (T)(a * (double)src[r]);
+}
+
+__global__ void k_axpby(double *__restrict__ y, const double *__restrict__ x, double a, double b, int n)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) y[r] = a * x[r] + b * y[r];
+}
+
+template <typename A, typename B>
+__global__ void k_convert(A *__restrict__ dst, const B *__restrict__ src, size_t n)
+{
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) dst[r] = (A)src[r];
+}
+
+// FP32 SpMV on the slot values (same structure as the FP64 one)
+__global__ void __launch_bounds__(256) k_spmv32(Scalars s, const float *__restrict__ J,
+                                                const float *__restrict__ x, float *__restrict__ y)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= s.ncell) return;
+    const int i = cell % s.n + 1, j = (cell / s.n) % s.m + 1, k = cell / (s.n * s.m) + 1;
+    const size_t C = (size_t)s.ncell;
+    float acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; sl++) {
+        int kk = SLOTS[sl].kk;
+        int ni = i + kk_di(kk), nj = j + kk_dj(kk), nk = k + kk_dk(kk);
+        if (s.periodic) ni = (ni < 1) ? ni + s.n : (ni > s.n ? ni - s.n : ni);
+        if (ni < 1 || ni > s.n || nj < 1 || nj > s.m || nk < 1 || nk > s.l) continue;
+        int nb = (ni - 1) + s.n * ((nj - 1) + s.m * (nk - 1));
+        acc[SLOTS[sl].eq] += J[(size_t)sl * C + cell] * x[6 * nb + SLOTS[sl].var];
+    }
+#pragma unroll
+    for (int e = 0; e < 6; e++) y[6 * (size_t)cell + e] = acc[e];
+}
+
+// integral row of the FP32 product (fixed-order two-stage reduction)
+__global__ void __launch_bounds__(256) k_dot32_partial(const float *__restrict__ a,
+                                                       const float *__restrict__ b, int n,
+                                                       double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        acc += (double)a[q] * (double)b[q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_dot32_final(const double *__restrict__ part, int np, float *__restrict__ out,
+                              double scale, int slot)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < np; q += blockDim.x) acc += part[q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[slot] = (float)(scale * sh[0]);
+}
+
+// ------------------------------------------------------------------ host
+static inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+void krylov_free(iemic_ctx *c)
+{
+    Krylov *k = c->kry;
+    if (!k) return;
+    void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->b32.V, k->b32.Z, k->b32.w, k->d_h, k->d_part,
+                    k->d_r, k->d_d, k->J32, k->coef32};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete k;
+    c->kry = nullptr;
+}
+
+template <typename T>
+static int ensure_basis(Basis<T> &B, int m, int n)
+{
+    if (B.cap >= m) return IEMIC_OK;
+    if (B.V) { (void)hipFree(B.V); (void)hipFree(B.Z); (void)hipFree(B.w); }
+    HIPCHK(hipMalloc(&B.V, (size_t)(m + 1) * n * sizeof(T)));
+    HIPCHK(hipMalloc(&B.Z, (size_t)m * n * sizeof(T)));
+    HIPCHK(hipMalloc(&B.w, (size_t)n * sizeof(T)));
+    B.cap = m;
+    return IEMIC_OK;
+}
+
+static int ensure_krylov(iemic_ctx *c, int m)
+{
+    if (!c->kry) c->kry = new Krylov();
+    Krylov *k = c->kry;
+    if (k->hcap < m) {
+        if (k->d_h) { (void)hipFree(k->d_h); (void)hipFree(k->d_part); }
+        k->nblocks = 480;
+        HIPCHK(hipMalloc(&k->d_h, (size_t)(m + 2) * sizeof(double)));
+        HIPCHK(hipMalloc(&k->d_part, (size_t)(m + 2) * k->nblocks * sizeof(double)));
+        k->hcap = m;
+    }
+    if (!k->d_r) {
+        HIPCHK(hipMalloc(&k->d_r, (size_t)c->sc.ndim * sizeof(double)));
+        HIPCHK(hipMalloc(&k->d_d, (size_t)c->sc.ndim * sizeof(double)));
+    }
+    return IEMIC_OK;
+}
+
+// matrix and preconditioner in the working precision T
+template <typename T> struct Ops;
+template <> struct Ops<double> {
+    static int matvec(iemic_ctx *c, const double *x, double *y) { return launch_spmv(c, x, y); }
+    static int prec(iemic_ctx *c, const double *x, double *y) { return precond_apply(c, x, y); }
+};
+template <> struct Ops<float> {
+    static int matvec(iemic_ctx *c, const float *x, float *y)
+    {
+        Krylov *k = c->kry;
+        hipLaunchKernelGGL(k_spmv32, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc, k->J32, x, y);
+        if (c->sc.rowintcon >= 0) {
+            int nb = std::min(c->nred, nblk(c->sc.ndim, 256));
+            hipLaunchKernelGGL(k_dot32_partial, dim3(nb), dim3(256), 0, c->stream, k->coef32, x, c->sc.ndim, c->d_red);
+            hipLaunchKernelGGL(k_dot32_final, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, y,
+                               (double)c->cfg.int_sign, c->sc.rowintcon);
+        }
+        HIPCHK(hipGetLastError());
+        return IEMIC_OK;
+    }
+    static int prec(iemic_ctx *c, const float *x, float *y) { return precond_apply_f32(c, x, y); }
+};
+
+// ||v||_2 via the multidot machinery (one vector against itself)
+template <typename T>
+static int norm2(iemic_ctx *c, const T *v, int n, double &out)
+{
+    Krylov *k = c->kry;
+    hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, v, (size_t)n, 1, v, n, k->d_part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, k->d_part, k->nblocks, 1, k->d_h + k->hcap + 1, 0);
+    double r;
+    HIPCHK(hipMemcpyAsync(&r, k->d_h + k->hcap + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    out = std::sqrt(r);
+    return IEMIC_OK;
+}
+
+// One FGMRES(m) solve of A x = b in precision T with x0 = 0, stopping when the
+// recurrence residual drops below tol*||b|| or after maxit iterations.
+// Returns iterations used in *its.
+template <typename T>
+static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, int verbose, int *its,
+                  double *est)
+{
+    Krylov *k = c->kry;
+    Basis<T> &B = std::is_same<T, float>::value ? *reinterpret_cast<Basis<T> *>(&k->b32)
+                                                : *reinterpret_cast<Basis<T> *>(&k->b64);
+    const int n = c->sc.ndim;
+    int rc = ensure_basis(B, m, n);
+    if (rc) return rc;
+    const size_t ld = n;
+    double bnorm;
+    rc = norm2(c, b, n, bnorm);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(x, 0, (size_t)n * sizeof(T), c->stream));
+    *its = 0;
+    *est = 1.0;
+    if (bnorm == 0.0) { *est = 0.0; return IEMIC_OK; }
+    std::vector<double> H((size_t)(m + 1) * m), cs(m + 1), sn(m + 1), s(m + 1), h(m + 2), y(m + 1);
+    // r0 = b (x0 = 0), later r = b - A x
+    T *r = B.w;
+    HIPCHK(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    double beta = bnorm;
+    int total = 0;
+    while (true) {
+        hipLaunchKernelGGL(k_scale_copy<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.V, r, 1.0 / beta, n);
+        std::fill(s.begin(), s.end(), 0.0);
+        s[0] = beta;
+        int j = 0;
+        bool done = false;
+        for (; j < m && total < maxit; j++, total++) {
+            T *vj = B.V + (size_t)j * ld, *zj = B.Z + (size_t)j * ld;
+            if ((rc = Ops<T>::prec(c, vj, zj))) return rc;
+            if ((rc = Ops<T>::matvec(c, zj, B.w))) return rc;
+            // two passes of classical Gram-Schmidt
+            for (int pass = 0; pass < 2; pass++) {
+                hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
+                hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1,
+                                   k->d_h, 0);
+                hipLaunchKernelGGL(k_update<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.w, B.V, ld, j + 1,
+                                   k->d_h, -1.0, n);
+                std::vector<double> hp(j + 1);
+                HIPCHK(hipMemcpyAsync(hp.data(), k->d_h, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                for (int q = 0; q <= j; q++) h[q] = pass ? h[q] + hp[q] : hp[q];
+            }
+            double hn;
+            if ((rc = norm2(c, B.w, n, hn))) return rc;
+            h[j + 1] = hn;
+            T *vn = B.V + (size_t)(j + 1) * ld;
+            hipLaunchKernelGGL(k_scale_copy<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, vn, B.w,
+                               hn > 0 ? 1.0 / hn : 0.0, n);
+            // Givens rotations on column j (GMRESSolver.H:190-200)
+            for (int q = 0; q < j; q++) {
+                double t = cs[q] * h[q] + sn[q] * h[q + 1];
+                h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+                h[q] = t;
+            }
+            double den = std::hypot(h[j], h[j + 1]);
+            cs[j] = den == 0.0 ? 1.0 : h[j] / den;
+            sn[j] = den == 0.0 ? 0.0 : h[j + 1] / den;
+            h[j] = cs[j] * h[j] + sn[j] * h[j + 1];
+            h[j + 1] = 0.0;
+            s[j + 1] = -sn[j] * s[j];
+            s[j] = cs[j] * s[j];
+            for (int q = 0; q <= j; q++) H[(size_t)q * m + j] = h[q];
+            *est = std::fabs(s[j + 1]) / bnorm;
+            if (verbose > 1 && (total % 10 == 0)) std::printf("  fgmres it %d est %.3e\n", total, *est);
+            if (*est < tol) {
+                j++;
+                total++;
+                done = true;
+                break;
+            }
+        }
+        // x += Z y with y = H^{-1} s (back substitution)
+        for (int q = j - 1; q >= 0; q--) {
+            double acc = s[q];
+            for (int p = q + 1; p < j; p++) acc -= H[(size_t)q * m + p] * y[p];
+            y[q] = acc / H[(size_t)q * m + q];
+        }
+        HIPCHK(hipMemcpyAsync(k->d_h, y.data(), j * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_update<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, x, B.Z, ld, j, k->d_h, 1.0, n);
+        *its = total;
+        if (done || total >= maxit) break;
+        // restart: r = b - A x
+        if ((rc = Ops<T>::matvec(c, x, B.w))) return rc;
+        hipLaunchKernelGGL(k_scale_copy<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.w, B.w, -1.0, n);
+        std::vector<double> one(1, 1.0);
+        // w = b - A x  (w holds -A x)
+        HIPCHK(hipMemcpyAsync(k->d_h, one.data(), sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_update<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.w, b, ld, 1, k->d_h, 1.0, n);
+        r = B.w;
+        if ((rc = norm2(c, r, n, beta))) return rc;
+        *est = beta / bnorm;
+        if (verbose) std::printf("  fgmres restart at %d: true residual %.3e\n", total, *est);
+        if (*est < tol) break;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IEMIC_OK;
+}
+
+// FP64 true residual r = b - J x and its norm ratio
+static int true_residual(iemic_ctx *c, const double *b, const double *x, double *r, double bnorm, double *rel)
+{
+    int rc = launch_spmv(c, x, r);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(c->sc.ndim, 256)), dim3(256), 0, c->stream, r, b, 1.0, -1.0, c->sc.ndim);
+    double rn;
+    if ((rc = norm2(c, r, c->sc.ndim, rn))) return rc;
+    *rel = bnorm > 0 ? rn / bnorm : 0.0;
+    return IEMIC_OK;
+}
+
+int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
+                 iemic_solve_info *info)
+{
+    using clk = std::chrono::steady_clock;
+    int rc = ensure_krylov(c, std::max(p->restart, p->inner_restart));
+    if (rc) return rc;
+    Krylov *k = c->kry;
+    iemic_solve_info inf{};
+    auto t0 = clk::now();
+    if (p->precision == 0) {
+        rc = precond_build(c);
+    } else {
+        if (!k->J32) {
+            HIPCHK(hipMalloc(&k->J32, (size_t)NSLOT * c->sc.ncell * sizeof(float)));
+            HIPCHK(hipMalloc(&k->coef32, (size_t)c->sc.ndim * sizeof(float)));
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(c->sc.ndim, 256)), dim3(256), 0, c->stream,
+                               k->coef32, c->d_coef, (size_t)c->sc.ndim);
+        }
+        size_t nj = (size_t)NSLOT * c->sc.ncell;
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nj, 256)), dim3(256), 0, c->stream, k->J32, c->d_J, nj);
+        rc = precond_build_f32(c);
+    }
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto t1 = clk::now();
+    inf.t_precon_setup = std::chrono::duration<double>(t1 - t0).count();
+    double bnorm;
+    if ((rc = norm2(c, d_b, c->sc.ndim, bnorm))) return rc;
+    int n = c->sc.ndim;
+    if (p->precision == 0) {
+        int its;
+        double est;
+        rc = fgmres<double>(c, d_b, d_x, p->restart, p->tol, p->maxit, p->verbose, &its, &est);
+        if (rc) return rc;
+        inf.iterations = its;
+        if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &inf.rel_residual))) return rc;
+    } else {
+        // FP64 iterative refinement around an FP32 inner FGMRES
+        HIPCHK(hipMemsetAsync(d_x, 0, (size_t)n * sizeof(double), c->stream));
+        HIPCHK(hipMemcpyAsync(k->d_r, d_b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        double rel = 1.0;
+        int total = 0, steps = 0;
+        float *r32 = nullptr, *d32 = nullptr;
+        HIPCHK(hipMalloc(&r32, (size_t)n * sizeof(float)));
+        HIPCHK(hipMalloc(&d32, (size_t)n * sizeof(float)));
+        while (rel > p->tol && total < p->maxit && steps < 50) {
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n, 256)), dim3(256), 0, c->stream, r32, k->d_r,
+                               (size_t)n);
+            int its;
+            double est;
+            rc = fgmres<float>(c, r32, d32, p->inner_restart, p->inner_tol, p->maxit - total, p->verbose, &its, &est);
+            if (rc) break;
+            total += its;
+            steps++;
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n, 256)), dim3(256), 0, c->stream, k->d_d, d32,
+                               (size_t)n);
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, d_x, k->d_d, 1.0, 1.0, n);
+            if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &rel))) break;
+            if (p->verbose) std::printf(" refinement %d: inner its %d, FP64 residual %.3e\n", steps, its, rel);
+        }
+        (void)hipFree(r32);
+        (void)hipFree(d32);
+        if (rc) return rc;
+        inf.iterations = total;
+        inf.refinements = steps;
+        inf.rel_residual = rel;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    inf.t_solve = std::chrono::duration<double>(clk::now() - t1).count();
+    inf.converged = inf.rel_residual <= p->tol;
+    if (info) *info = inf;
+    return inf.converged ? IEMIC_OK : set_error(IEMIC_ENOCONV, "solver did not reach the tolerance");
+}
+
+}  // namespace iemic

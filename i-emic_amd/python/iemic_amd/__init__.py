@@ -1,0 +1,314 @@
+"""ctypes binding of libiemic_amd.so (include/iemic_amd.h).
+
+`Ocean` mirrors the reference's Model/Ocean facade (src/utils/Model.H:55-113,
+src/ocean/Ocean.H:175): computeRHS, computeJacobian, applyMatrix,
+applyMassMat, buildPreconditioner, applyPrecon, solve, getPar/setPar.  Vectors
+cross the boundary as float64 numpy arrays in the reference's unknown order;
+the *_dev entry points are reachable through `lib` for device-resident
+callers.  Loading fails loudly when the HIP library is missing: there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "libiemic_amd.so"))
+
+PAR_INDEX = {
+    "AL_T": 1, "Rayleigh-Number": 2, "Vertical Ekman-Number": 3, "Horizontal Ekman-Number": 4,
+    "Rossby-Number": 5, "MIXP": 6, "RESC": 7, "SPL1": 8, "Salinity Homotopy": 9,
+    "Solar Forcing": 10, "Horizontal Peclet-Number": 11, "Vertical Peclet-Number": 12,
+    "P_VC": 13, "LAMB": 14, "Salinity Forcing": 15, "Wind Forcing": 16,
+    "Temperature Forcing": 17, "Nonlinear Factor": 18, "Combined Forcing": 19, "ARCL": 20,
+    "NLES": 21, "IFRICB": 22, "CONT": 23, "Energy": 24, "ALPC": 25, "CMPR": 26,
+    "Flux Perturbation": 27, "Salinity Perturbation": 28, "MKAP": 29, "SPL2": 30,
+}
+# short Fortran names of par.F90:32-67
+PAR_SHORT = dict(AL_T=1, RAYL=2, EK_V=3, EK_H=4, ROSB=5, MIXP=6, RESC=7, SPL1=8, HMTP=9,
+                 SUNP=10, PE_H=11, PE_V=12, P_VC=13, LAMB=14, SALT=15, WIND=16, TEMP=17,
+                 BIOT=18, COMB=19, ARCL=20, NLES=21, IFRICB=22, CONT=23, ENER=24, ALPC=25,
+                 CMPR=26, FPER=27, SPER=28, MKAP=29, SPL2=30)
+
+
+class Config(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("l", C.c_int),
+                ("xmin_deg", C.c_double), ("xmax_deg", C.c_double),
+                ("ymin_deg", C.c_double), ("ymax_deg", C.c_double),
+                ("periodic", C.c_int), ("hdim", C.c_double), ("qz", C.c_double),
+                ("alphaT", C.c_double), ("alphaS", C.c_double),
+                ("ih", C.c_int), ("vmix", C.c_int), ("tap", C.c_int),
+                ("rho_mixing", C.c_int), ("coriolis_on", C.c_int),
+                ("tres", C.c_int), ("sres", C.c_int), ("iza", C.c_int), ("ite", C.c_int),
+                ("its", C.c_int), ("coupled_t", C.c_int), ("coupled_s", C.c_int),
+                ("forcing_type", C.c_int), ("int_sign", C.c_int), ("nic", C.c_int),
+                ("mic", C.c_int)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("restart", C.c_int), ("maxit", C.c_int),
+                ("precision", C.c_int), ("inner_restart", C.c_int), ("inner_tol", C.c_double),
+                ("verbose", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("rel_residual", C.c_double),
+                ("t_precon_setup", C.c_double), ("t_solve", C.c_double), ("converged", C.c_int)]
+
+
+_P = C.POINTER
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built (run i-emic_amd/build.py)")
+    lib = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, _P(C.c_double), _P(C.c_int)
+    sig = {
+        "iemic_last_error": ([], C.c_char_p),
+        "iemic_device_count": ([ip], C.c_int),
+        "iemic_create": ([_P(Config), ip, dp, dp, dp, dp, dp, _P(vp)], C.c_int),
+        "iemic_destroy": ([vp], C.c_int),
+        "iemic_set_par": ([vp, C.c_int, C.c_double], C.c_int),
+        "iemic_get_par": ([vp, C.c_int, dp], C.c_int),
+        "iemic_ndim": ([vp, ip], C.c_int),
+        "iemic_graph_nnz": ([vp, ip], C.c_int),
+        "iemic_rowintcon": ([vp, ip], C.c_int),
+        "iemic_intcond_coef": ([vp, dp], C.c_int),
+        "iemic_forcing": ([vp, dp], C.c_int),
+        "iemic_set_intcond_correction": ([vp, dp], C.c_int),
+        "iemic_set_intcond_correction_value": ([vp, C.c_double], C.c_int),
+        "iemic_rhs": ([vp, dp, dp], C.c_int),
+        "iemic_rhs_dev": ([vp, vp, vp], C.c_int),
+        "iemic_jacobian": ([vp, dp], C.c_int),
+        "iemic_jacobian_dev": ([vp, vp], C.c_int),
+        "iemic_jacobian_csr": ([vp, ip, ip, dp, dp], C.c_int),
+        "iemic_apply_matrix": ([vp, dp, dp], C.c_int),
+        "iemic_apply_matrix_dev": ([vp, vp, vp], C.c_int),
+        "iemic_apply_matrix_transpose_dev": ([vp, vp, vp], C.c_int),
+        "iemic_apply_mass": ([vp, dp, dp], C.c_int),
+        "iemic_build_preconditioner": ([vp], C.c_int),
+        "iemic_apply_precon": ([vp, dp, dp], C.c_int),
+        "iemic_apply_precon_dev": ([vp, vp, vp], C.c_int),
+        "iemic_solver_defaults": ([_P(SolverParams)], None),
+        "iemic_solve": ([vp, _P(SolverParams), dp, dp, _P(SolveInfo)], C.c_int),
+        "iemic_solve_dev": ([vp, _P(SolverParams), vp, vp, _P(SolveInfo)], C.c_int),
+        "iemic_dev_alloc": ([_P(vp), C.c_longlong], C.c_int),
+        "iemic_dev_free": ([vp], C.c_int),
+        "iemic_memcpy_h2d": ([vp, vp, C.c_longlong], C.c_int),
+        "iemic_memcpy_d2h": ([vp, vp, C.c_longlong], C.c_int),
+        "iemic_synchronize": ([vp], C.c_int),
+        "iemic_last_kernel_ms": ([vp, C.c_int, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+class IemicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"iemic error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc):
+    if rc != 0:
+        raise IemicError(rc, load().iemic_last_error().decode())
+    return rc
+
+
+def _d(a):
+    return a.ctypes.data_as(_P(C.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DeviceArray:
+    """A float64 device buffer owned by the library's allocator."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        _check(load().iemic_dev_alloc(C.byref(self.ptr), 8 * self.n))
+
+    def upload(self, a):
+        a = _f64(a)
+        assert a.size == self.n
+        _check(load().iemic_memcpy_h2d(self.ptr, a.ctypes.data_as(C.c_void_p), 8 * self.n))
+        return self
+
+    def download(self):
+        out = np.empty(self.n)
+        _check(load().iemic_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, 8 * self.n))
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().iemic_dev_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Ocean:
+    """One THCM ocean model on the GPU (the reference's Ocean/THCM pair)."""
+
+    def __init__(self, cfg, landm, taux=None, tauy=None, tatm=None, emip=None, spert=None,
+                 pars=None):
+        lib = load()
+        self.lib = lib
+        self.cfg = cfg
+        landm = np.ascontiguousarray(landm, dtype=np.int32)
+        fields = [None if f is None else _f64(f) for f in (taux, tauy, tatm, emip, spert)]
+        self._keep = fields
+        h = C.c_void_p()
+        _check(lib.iemic_create(C.byref(cfg), landm.ctypes.data_as(_P(C.c_int)),
+                                *[None if f is None else _d(f) for f in fields], C.byref(h)))
+        self.h = h
+        n = C.c_int()
+        _check(lib.iemic_ndim(h, C.byref(n)))
+        self.ndim = n.value
+        for k, v in (pars or {}).items():
+            self.setPar(k, v)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.iemic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- parameters (THCM::setParameter / getParameter)
+    @staticmethod
+    def _idx(name):
+        if isinstance(name, int):
+            return name
+        return PAR_INDEX.get(name, PAR_SHORT.get(name))
+
+    def setPar(self, name, value):
+        _check(self.lib.iemic_set_par(self.h, self._idx(name), float(value)))
+
+    def getPar(self, name):
+        v = C.c_double()
+        _check(self.lib.iemic_get_par(self.h, self._idx(name), C.byref(v)))
+        return v.value
+
+    # ---- structure
+    def rowintcon(self):
+        r = C.c_int()
+        _check(self.lib.iemic_rowintcon(self.h, C.byref(r)))
+        return r.value
+
+    def graph_nnz(self):
+        r = C.c_int()
+        _check(self.lib.iemic_graph_nnz(self.h, C.byref(r)))
+        return r.value
+
+    def intcond_coef(self):
+        out = np.zeros(self.ndim)
+        _check(self.lib.iemic_intcond_coef(self.h, _d(out)))
+        return out
+
+    def forcing(self):
+        out = np.zeros(self.ndim)
+        _check(self.lib.iemic_forcing(self.h, _d(out)))
+        return out
+
+    def setIntCondCorrection(self, x=None, value=None):
+        if x is not None:
+            x = _f64(x)
+            _check(self.lib.iemic_set_intcond_correction(self.h, _d(x)))
+        else:
+            _check(self.lib.iemic_set_intcond_correction_value(self.h, float(value)))
+
+    # ---- Model interface
+    def computeRHS(self, x):
+        x = _f64(x)
+        F = np.empty(self.ndim)
+        _check(self.lib.iemic_rhs(self.h, _d(x), _d(F)))
+        return F
+
+    def computeJacobian(self, x):
+        x = _f64(x)
+        _check(self.lib.iemic_jacobian(self.h, _d(x)))
+
+    def getJacobian(self):
+        """(rowptr, col, val, diagB) in the reference's CSR order."""
+        nnz = self.graph_nnz()
+        rp = np.empty(self.ndim + 1, np.int32)
+        ci = np.empty(nnz, np.int32)
+        val = np.empty(nnz)
+        dB = np.empty(self.ndim)
+        _check(self.lib.iemic_jacobian_csr(self.h, rp.ctypes.data_as(_P(C.c_int)),
+                                           ci.ctypes.data_as(_P(C.c_int)), _d(val), _d(dB)))
+        return rp, ci, val, dB
+
+    def applyMatrix(self, v):
+        v = _f64(v)
+        out = np.empty(self.ndim)
+        _check(self.lib.iemic_apply_matrix(self.h, _d(v), _d(out)))
+        return out
+
+    def applyMatrixTranspose(self, v):
+        dv = DeviceArray(self.ndim).upload(v)
+        do = DeviceArray(self.ndim)
+        _check(self.lib.iemic_apply_matrix_transpose_dev(self.h, dv.ptr, do.ptr))
+        _check(self.lib.iemic_synchronize(self.h))
+        return do.download()
+
+    def applyMassMat(self, v):
+        v = _f64(v)
+        out = np.empty(self.ndim)
+        _check(self.lib.iemic_apply_mass(self.h, _d(v), _d(out)))
+        return out
+
+    def buildPreconditioner(self):
+        _check(self.lib.iemic_build_preconditioner(self.h))
+
+    def applyPrecon(self, v):
+        v = _f64(v)
+        out = np.empty(self.ndim)
+        _check(self.lib.iemic_apply_precon(self.h, _d(v), _d(out)))
+        return out
+
+    @staticmethod
+    def solver_params(**kw):
+        p = SolverParams()
+        load().iemic_solver_defaults(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def solve(self, b, params=None, raise_on_noconv=True):
+        """Ocean::solve: J x = b from x0 = 0; returns (x, SolveInfo)."""
+        b = _f64(b)
+        x = np.zeros(self.ndim)
+        info = SolveInfo()
+        p = params if params is not None else self.solver_params()
+        rc = self.lib.iemic_solve(self.h, C.byref(p), _d(b), _d(x), C.byref(info))
+        if rc != 0 and (raise_on_noconv or rc != -6):
+            _check(rc)
+        return x, info
+
+    def kernel_ms(self, which):
+        v = C.c_double()
+        _check(self.lib.iemic_last_kernel_ms(self.h, which, C.byref(v)))
+        return v.value

@@ -1,0 +1,151 @@
+/*
+ * iemic_amd -- MI355X (gfx950) implementation of the I-EMIC implicit solve
+ * path: THCM residual F(x) and Jacobian J(x) assembly, matrix/mass-matrix
+ * application, preconditioner and right-preconditioned FGMRES.
+ *
+ * C ABI: plain pointers and sizes only.  One context = one THCM ocean model
+ * (the reference keeps THCM as a process-wide singleton, THCM.H:76-84; here
+ * any number of contexts may coexist, one model instance per context).
+ *
+ * Conventions shared with the reference:
+ *  - unknown ordering: row = 6*((k*M + j)*N + i) + XX, XX = 0..5 for
+ *    u,v,w,p,T,S, i fastest (THCMdefs.H FIND_ROW2, 0-based here);
+ *  - F is the negated THCM right-hand side with the salinity integral
+ *    condition in row `rowintcon` (THCM.C:1011-1027);
+ *  - J lives on the fixed maximal graph of THCM::CreateMaximalGraph
+ *    (THCM.C:2300-2580); iemic_jacobian_csr exports it in that row/column
+ *    order;
+ *  - land mask: (N+2)(M+2)(L+2) ints, i fastest, OCEAN=0 LAND=1 PERIO=3, as
+ *    handed to the Fortran init_ (usrc.F90:6-31).
+ * Functions ending in _dev take device pointers (hipMalloc'ed, resident in
+ * HBM); the others take host pointers and copy.
+ * Every function returns 0 on success or a negative IEMIC_E* code; the
+ * message of the last error is available from iemic_last_error().
+ */
+#ifndef IEMIC_AMD_H
+#define IEMIC_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    IEMIC_OK = 0,
+    IEMIC_EINVAL = -1,     /* bad argument or configuration            */
+    IEMIC_EUNSUPPORTED = -2, /* option outside this implementation     */
+    IEMIC_EHIP = -3,       /* HIP runtime error                         */
+    IEMIC_ENOMEM = -4,     /* device allocation failed                  */
+    IEMIC_ESTATE = -5,     /* call order (e.g. no Jacobian yet)         */
+    IEMIC_ENOCONV = -6     /* solver did not reach its tolerance        */
+};
+
+/* THCM initialisation parameters (THCM::THCM, THCM.C:180-260, and the
+ * Fortran init_ arguments, usrc.F90:6-31).  Degrees for the bounds. */
+typedef struct {
+    int n, m, l;
+    double xmin_deg, xmax_deg, ymin_deg, ymax_deg;
+    int periodic;
+    double hdim, qz;
+    double alphaT, alphaS;
+    int ih, vmix, tap, rho_mixing, coriolis_on;
+    int tres, sres, iza, ite, its, coupled_t, coupled_s, forcing_type;
+    int int_sign, nic, mic;
+} iemic_config;
+
+typedef struct iemic_ctx iemic_ctx;
+
+/* Solver controls (Ocean::getDefaultInitParameters, Ocean.C:2264-2301, and
+ * GMRESSolver::setParameters, GMRESSolver.H:52-68). */
+typedef struct {
+    double tol;          /* relative residual ||b - J x|| / ||b||            */
+    int restart;         /* Krylov space size (Belos "Num Blocks")           */
+    int maxit;           /* total iteration cap                              */
+    int precision;       /* 0: FP64 throughout; 1: FP32 inner solver with     */
+                         /*    FP64 iterative refinement of the outer residual */
+    int inner_restart;   /* mixed precision: inner FGMRES(m)                  */
+    double inner_tol;    /* mixed precision: inner relative tolerance         */
+    int verbose;
+} iemic_solver_params;
+
+typedef struct {
+    int iterations;       /* total Krylov iterations (inner, in mixed mode)   */
+    int refinements;      /* outer FP64 refinement steps (mixed mode)         */
+    double rel_residual;  /* true FP64 ||b - J x|| / ||b||                    */
+    double t_precon_setup;/* seconds                                          */
+    double t_solve;       /* seconds                                          */
+    int converged;
+} iemic_solve_info;
+
+const char *iemic_last_error(void);
+int iemic_device_count(int *count);
+
+/* ---- lifecycle: THCM::THCM / Fortran init_ (usrc.F90:6-139) ------------ */
+/* taux..spert: N*M surface fields, i fastest (NULL = zero), the arrays the
+ * reference passes to init_ / m_inserts (inserts.F90:12-265). */
+int iemic_create(const iemic_config *cfg, const int *landm, const double *taux,
+                 const double *tauy, const double *tatm, const double *emip,
+                 const double *spert, iemic_ctx **out);
+int iemic_destroy(iemic_ctx *ctx);
+
+/* ---- parameters: setparcs_/getparcs_ (usrc.F90:143-180), 1..30 -------- */
+int iemic_set_par(iemic_ctx *ctx, int idx, double value);
+int iemic_get_par(const iemic_ctx *ctx, int idx, double *value);
+
+/* ---- sizes and fixed structure ----------------------------------------- */
+int iemic_ndim(const iemic_ctx *ctx, int *ndim);
+int iemic_graph_nnz(const iemic_ctx *ctx, int *nnz);         /* CreateMaximalGraph */
+int iemic_rowintcon(const iemic_ctx *ctx, int *row);         /* THCM.C:695, -1 if SRES=1 */
+int iemic_intcond_coef(const iemic_ctx *ctx, double *coef);  /* getIntCondCoeff, THCM.C:2608 */
+int iemic_forcing(const iemic_ctx *ctx, double *frc);        /* THCM::getForcing, THCM.C:944 */
+/* THCM::setIntCondCorrection (THCM.C:2079-2097): intCorrection = coef . x */
+int iemic_set_intcond_correction(iemic_ctx *ctx, const double *x);
+int iemic_set_intcond_correction_value(iemic_ctx *ctx, double value);
+
+/* ---- Ocean::computeRHS (Ocean.C:1277-1284) ------------------------------ */
+int iemic_rhs(iemic_ctx *ctx, const double *x, double *F);
+int iemic_rhs_dev(iemic_ctx *ctx, const double *d_x, double *d_F);
+
+/* ---- Ocean::computeJacobian (Ocean.C:1297-1309); J stays on the device -- */
+int iemic_jacobian(iemic_ctx *ctx, const double *x);
+int iemic_jacobian_dev(iemic_ctx *ctx, const double *d_x);
+/* export J in the reference's CSR (row order, sorted columns of the fixed
+ * graph) and diag(B) of THCM::evaluate (THCM.C:1157-1169); arrays of size
+ * ndim+1, graph_nnz, graph_nnz, ndim. */
+int iemic_jacobian_csr(iemic_ctx *ctx, int *rowptr, int *col, double *val, double *diagB);
+
+/* ---- Ocean::applyMatrix / applyMassMat (Ocean.C:1369-1374, 1448-1457) -- */
+int iemic_apply_matrix(iemic_ctx *ctx, const double *v, double *out);
+int iemic_apply_matrix_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
+/* transpose product out = J^T v (no counterpart in the reference: its
+ * TRIOS preconditioner rejects SetUseTranspose, TRIOS_BlockPreconditioner.H:122;
+ * used by the adjoint solve). */
+int iemic_apply_matrix_transpose_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
+int iemic_apply_mass(iemic_ctx *ctx, const double *v, double *out);
+
+/* ---- Ocean::buildPreconditioner / applyPrecon (Ocean.C:1377-1416) ------ */
+int iemic_build_preconditioner(iemic_ctx *ctx);
+int iemic_apply_precon(iemic_ctx *ctx, const double *v, double *out);
+int iemic_apply_precon_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
+
+/* ---- Ocean::solve (Ocean.C:1070-1147): J x = b, x0 = 0 ------------------ */
+void iemic_solver_defaults(iemic_solver_params *p);
+int iemic_solve(iemic_ctx *ctx, const iemic_solver_params *p, const double *b, double *x,
+                iemic_solve_info *info);
+int iemic_solve_dev(iemic_ctx *ctx, const iemic_solver_params *p, const double *d_b,
+                    double *d_x, iemic_solve_info *info);
+
+/* ---- device memory helpers for callers without their own allocator ----- */
+int iemic_dev_alloc(void **ptr, long long bytes);
+int iemic_dev_free(void *ptr);
+int iemic_memcpy_h2d(void *dst, const void *src, long long bytes);
+int iemic_memcpy_d2h(void *dst, const void *src, long long bytes);
+int iemic_synchronize(iemic_ctx *ctx);
+
+/* ---- timing of the last device launches (HIP events, milliseconds) ----- */
+/* kernel: 0 = F assembly, 1 = J assembly, 2 = SpMV, 3 = preconditioner apply */
+int iemic_last_kernel_ms(const iemic_ctx *ctx, int kernel, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
