@@ -210,7 +210,7 @@ static int ensure_krylov(iemic_ctx *c, int m)
     if (k->hcap < m) {
         if (k->d_h) { (void)hipFree(k->d_h); (void)hipFree(k->d_part); }
         k->nblocks = 480;
-        HIPCHK(hipMalloc(&k->d_h, (size_t)(m + 2) * sizeof(double)));
+        HIPCHK(hipMalloc(&k->d_h, (size_t)2 * (m + 2) * sizeof(double) + 64));
         HIPCHK(hipMalloc(&k->d_part, (size_t)(m + 2) * k->nblocks * sizeof(double)));
         k->hcap = m;
     }
@@ -250,17 +250,24 @@ static int norm2(iemic_ctx *c, const T *v, int n, double &out)
 {
     Krylov *k = c->kry;
     hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, v, (size_t)n, 1, v, n, k->d_part);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, k->d_part, k->nblocks, 1, k->d_h + k->hcap + 1, 0);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, k->d_part, k->nblocks, 1, k->d_h + 2 * (k->hcap + 2), 0);
     double r;
-    HIPCHK(hipMemcpyAsync(&r, k->d_h + k->hcap + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&r, k->d_h + 2 * (k->hcap + 2), sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     out = std::sqrt(r);
     return IEMIC_OK;
 }
 
-// One FGMRES(m) solve of A x = b in precision T with x0 = 0, stopping when the
-// recurrence residual drops below tol*||b|| or after maxit iterations.
-// Returns iterations used in *its.
+__global__ void k_add_into(double *__restrict__ a, const double *__restrict__ b, int n)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) a[q] += b[q];
+}
+
+// One FGMRES(m) solve of A x = b in precision T from x0 = 0.  The recurrence
+// residual is trusted only to decide when to stop a cycle: convergence is
+// declared on the true residual ||b - A x|| <= tol ||b||, computed after every
+// cycle (explicit residual check of GMRESSolver.H:229-248).
 template <typename T>
 static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, int verbose, int *its,
                   double *est)
@@ -280,39 +287,37 @@ static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, 
     *est = 1.0;
     if (bnorm == 0.0) { *est = 0.0; return IEMIC_OK; }
     std::vector<double> H((size_t)(m + 1) * m), cs(m + 1), sn(m + 1), s(m + 1), h(m + 2), y(m + 1);
-    // r0 = b (x0 = 0), later r = b - A x
+    double *d_h = k->d_h, *d_h2 = k->d_h + (m + 2);
     T *r = B.w;
     HIPCHK(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     double beta = bnorm;
     int total = 0;
+    const int nb = nblk(n, 256);
     while (true) {
-        hipLaunchKernelGGL(k_scale_copy<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.V, r, 1.0 / beta, n);
+        hipLaunchKernelGGL(k_scale_copy<T>, dim3(nb), dim3(256), 0, c->stream, B.V, r, 1.0 / beta, n);
         std::fill(s.begin(), s.end(), 0.0);
         s[0] = beta;
         int j = 0;
-        bool done = false;
         for (; j < m && total < maxit; j++, total++) {
             T *vj = B.V + (size_t)j * ld, *zj = B.Z + (size_t)j * ld;
             if ((rc = Ops<T>::prec(c, vj, zj))) return rc;
             if ((rc = Ops<T>::matvec(c, zj, B.w))) return rc;
-            // two passes of classical Gram-Schmidt
-            for (int pass = 0; pass < 2; pass++) {
-                hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
-                hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1,
-                                   k->d_h, 0);
-                hipLaunchKernelGGL(k_update<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.w, B.V, ld, j + 1,
-                                   k->d_h, -1.0, n);
-                std::vector<double> hp(j + 1);
-                HIPCHK(hipMemcpyAsync(hp.data(), k->d_h, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(hipStreamSynchronize(c->stream));
-                for (int q = 0; q <= j; q++) h[q] = pass ? h[q] + hp[q] : hp[q];
-            }
-            double hn;
-            if ((rc = norm2(c, B.w, n, hn))) return rc;
+            // classical Gram-Schmidt, twice; coefficients summed on the device
+            hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
+            hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1, d_h, 0);
+            hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(256), 0, c->stream, B.w, B.V, ld, j + 1, d_h, -1.0, n);
+            hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
+            hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1, d_h2, 0);
+            hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(256), 0, c->stream, B.w, B.V, ld, j + 1, d_h2, -1.0, n);
+            hipLaunchKernelGGL(k_add_into, dim3(1), dim3(256), 0, c->stream, d_h, d_h2, j + 1);
+            hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.w, (size_t)n, 1, B.w, n, k->d_part);
+            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, k->d_part, k->nblocks, 1, d_h + j + 1, 0);
+            HIPCHK(hipMemcpyAsync(h.data(), d_h, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            double hn = std::sqrt(h[j + 1]);
             h[j + 1] = hn;
             T *vn = B.V + (size_t)(j + 1) * ld;
-            hipLaunchKernelGGL(k_scale_copy<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, vn, B.w,
-                               hn > 0 ? 1.0 / hn : 0.0, n);
+            hipLaunchKernelGGL(k_scale_copy<T>, dim3(nb), dim3(256), 0, c->stream, vn, B.w, hn > 0 ? 1.0 / hn : 0.0, n);
             // Givens rotations on column j (GMRESSolver.H:190-200)
             for (int q = 0; q < j; q++) {
                 double t = cs[q] * h[q] + sn[q] * h[q + 1];
@@ -329,10 +334,9 @@ static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, 
             for (int q = 0; q <= j; q++) H[(size_t)q * m + j] = h[q];
             *est = std::fabs(s[j + 1]) / bnorm;
             if (verbose > 1 && (total % 10 == 0)) std::printf("  fgmres it %d est %.3e\n", total, *est);
-            if (*est < tol) {
+            if (*est < tol || hn == 0.0) {
                 j++;
                 total++;
-                done = true;
                 break;
             }
         }
@@ -342,22 +346,20 @@ static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, 
             for (int p = q + 1; p < j; p++) acc -= H[(size_t)q * m + p] * y[p];
             y[q] = acc / H[(size_t)q * m + q];
         }
-        HIPCHK(hipMemcpyAsync(k->d_h, y.data(), j * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_update<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, x, B.Z, ld, j, k->d_h, 1.0, n);
+        HIPCHK(hipMemcpyAsync(d_h, y.data(), j * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(256), 0, c->stream, x, B.Z, ld, j, d_h, 1.0, n);
         *its = total;
-        if (done || total >= maxit) break;
-        // restart: r = b - A x
+        // explicit residual r = b - A x
         if ((rc = Ops<T>::matvec(c, x, B.w))) return rc;
-        hipLaunchKernelGGL(k_scale_copy<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.w, B.w, -1.0, n);
         std::vector<double> one(1, 1.0);
-        // w = b - A x  (w holds -A x)
-        HIPCHK(hipMemcpyAsync(k->d_h, one.data(), sizeof(double), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_update<T>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, B.w, b, ld, 1, k->d_h, 1.0, n);
+        HIPCHK(hipMemcpyAsync(d_h2, one.data(), sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_scale_copy<T>, dim3(nb), dim3(256), 0, c->stream, B.w, B.w, -1.0, n);
+        hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(256), 0, c->stream, B.w, b, ld, 1, d_h2, 1.0, n);
         r = B.w;
         if ((rc = norm2(c, r, n, beta))) return rc;
         *est = beta / bnorm;
-        if (verbose) std::printf("  fgmres restart at %d: true residual %.3e\n", total, *est);
-        if (*est < tol) break;
+        if (verbose) std::printf("  fgmres cycle end at %d: true residual %.3e\n", total, *est);
+        if (*est < tol || total >= maxit) break;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return IEMIC_OK;
