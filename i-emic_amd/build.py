@@ -22,7 +22,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-Wno-unused-variable"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
-SOURCES = ["host_setup.cpp", "capi.cpp", "assembly.hip", "precond.hip", "solver.hip"]
+SOURCES = ["host_setup.cpp", "capi.cpp", "assembly.hip", "precond.hip", "rowlu.hip", "solver.hip"]
+LIBS = ["-lrocsolver", "-lrocblas"]
 
 
 def _compile(src):
@@ -47,7 +48,7 @@ def build(verbose=False):
             print(r.stderr)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

@@ -991,6 +991,7 @@ int launch_jacobian(iemic_ctx *c, const double *d_x)
     timed_end(c, 1);
     HIPCHK(hipGetLastError());
     c->have_jac = true;
+    c->jac_version++;
     return IEMIC_OK;
 }
 

@@ -13,9 +13,11 @@ using namespace iemic;
 
 namespace iemic {
 const char *last_error();
-int precond_build(iemic_ctx *c);
-int precond_apply(iemic_ctx *c, const double *d_in, double *d_out);
+int precond_build(iemic_ctx *c, int precision);
+int precond_apply(iemic_ctx *c, int precision, const double *in, double *out, float *tmp32);
 void precond_free(iemic_ctx *c);
+int precond_check(iemic_ctx *c);
+int precond_info(iemic_ctx *c, iemic_precond_info *info);
 int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
                  iemic_solve_info *info);
 void krylov_free(iemic_ctx *c);
@@ -265,31 +267,51 @@ int iemic_apply_mass(iemic_ctx *c, const double *v, double *out)
     return IEMIC_OK;
 }
 
+int iemic_set_preconditioner(iemic_ctx *c, int kind)
+{
+    CHECK_CTX(c);
+    if (kind != IEMIC_PREC_ROW_LU && kind != IEMIC_PREC_COLUMN_ILU)
+        return set_error(IEMIC_EINVAL, "unknown preconditioner kind");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    precond_free(c);
+    c->prec_kind = kind;
+    return IEMIC_OK;
+}
+
+int iemic_preconditioner_info(iemic_ctx *c, iemic_precond_info *info)
+{
+    CHECK_CTX(c);
+    if (!info) return set_error(IEMIC_EINVAL, "null pointer");
+    return precond_info(c, info);
+}
+
 int iemic_build_preconditioner(iemic_ctx *c)
 {
     CHECK_CTX(c);
     if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
-    return precond_build(c);
+    return precond_build(c, 0);
 }
 
 int iemic_apply_precon_dev(iemic_ctx *c, const double *d_v, double *d_out)
 {
     CHECK_CTX(c);
-    return precond_apply(c, d_v, d_out);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    return precond_apply(c, 0, d_v, d_out, nullptr);
 }
 
 int iemic_apply_precon(iemic_ctx *c, const double *v, double *out)
 {
     CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
     int rc = ensure_staging(c);
     if (rc) return rc;
     size_t bytes = (size_t)c->sc.ndim * sizeof(double);
     HIPCHK(hipMemcpyAsync(c->d_x, v, bytes, hipMemcpyHostToDevice, c->stream));
-    rc = precond_apply(c, c->d_x, c->d_y);
+    rc = precond_apply(c, 0, c->d_x, c->d_y, nullptr);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(out, c->d_y, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return IEMIC_OK;
+    return precond_check(c);
 }
 
 void iemic_solver_defaults(iemic_solver_params *p)
@@ -299,8 +321,6 @@ void iemic_solver_defaults(iemic_solver_params *p)
     p->restart = 250;
     p->maxit = 1000;
     p->precision = 0;
-    p->inner_restart = 100;
-    p->inner_tol = 1e-3;
     p->verbose = 0;
 }
 

@@ -12,6 +12,7 @@ namespace iemic {
 
 struct Precond;   // preconditioner (precond.hip)
 struct Krylov;    // solver workspace (solver.hip)
+struct RowLU;     // row-block LU preconditioner (rowlu.hip)
 
 struct Host {
     // grid (grid.F90), Fortran lower bounds kept: x[1..n], y[0..m+1],
@@ -58,7 +59,10 @@ struct iemic_ctx {
     int *d_src = nullptr;            // CSR export map (slot index per entry)
     int *d_col_export = nullptr;     // CSR export columns
     bool have_jac = false;
+    unsigned jac_version = 0;        // bumped by every Jacobian assembly
+    int prec_kind = IEMIC_PREC_ROW_LU;
     iemic::Precond *prec = nullptr;
+    iemic::RowLU *rowlu = nullptr;
     iemic::Krylov *kry = nullptr;
     // timing
     hipEvent_t ev[8][2] = {};

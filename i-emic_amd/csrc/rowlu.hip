@@ -1,0 +1,489 @@
+// Row-block LU preconditioner: the default behind Ocean::buildPreconditioner
+// / applyPrecon (Ocean.C:1377-1416), in the slot where the reference uses the
+// TRIOS block preconditioner (TRIOS_BlockPreconditioner.C).
+//
+// The ocean unknowns of one zonal row j (all water columns i at that
+// latitude, all their wet levels, all six variables) form one dense block of
+// size n_j; local index = base(i) + 6 (k - k0(i)) + var.  The 27-point
+// stencil couples a row only to rows j - 1 and j + 1, so J restricted to the
+// ocean unknowns is block tridiagonal in j (the periodic wrap in i stays
+// inside the row blocks) and factors exactly by block elimination:
+//     D_0 = A_00,   X_j = D_{j-1}^{-1} A_{j-1,j},   D_j = A_jj - A_{j,j-1} X_j,
+//     Minv_j = D_j^{-1}  (LU with partial pivoting, then the explicit inverse).
+// The couplings A_{j,j-1}, A_{j-1,j} stay sparse (at most the stencil's
+// entries per unknown, read from the Jacobian slots through precomputed
+// index lists); Minv_j and X_j are dense, column-major.  Applying M^{-1} is a
+// forward sweep z_j = Minv_j (b_j - A_{j,j-1} z_{j-1}) and a backward sweep
+// z_{j-1} -= X_j z_j, i.e. 2 m - 1 dense matrix-vector products.
+//
+// The factored matrix differs from J in three ways, all absorbed by the
+// outer FGMRES: land rows are identity rows of J and pass through unchanged;
+// the dense integral-condition row is represented by its diagonal entry; and
+// one continuity row per pressure basin is replaced by p = 0 (pressure_pins),
+// which removes the pressure null vectors of J from the factors.
+//
+// Dense LU and inversion are rocSOLVER calls, the matrix-vector products
+// rocBLAS GEMV; the block assembly, the sparse Schur-complement updates and
+// the row gathers are kernels of this file.  The FP32 factors are computed
+// in FP32 (from the FP64 slot values) for the mixed-precision solve.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace iemic {
+
+std::vector<int> column_tops(const iemic_ctx *c);
+std::vector<uint8_t> pressure_pins(const iemic_ctx *c, const std::vector<int> &k0, const std::vector<int> &rank);
+
+template <typename T>
+struct RowFactors {
+    T *blk = nullptr;        // Minv_j and X_j
+    unsigned version = 0;    // Jacobian version the factors belong to
+    bool built = false;
+};
+
+struct RowLU {
+    int m = 0, nmax = 0, nocean = 0, npins = 0;
+    std::vector<int> nj, ubase;          // unknowns per row, first ocean unknown of the row
+    std::vector<size_t> doff, xoff;      // element offsets of Minv_j and X_j (j >= 1)
+    size_t total = 0;
+    int *d_loc = nullptr;                // per cell: local index of its first unknown in its row, -1 on land
+    int *d_perm = nullptr;               // per ocean unknown: model index 6 cell + var
+    int *d_lptr = nullptr;               // A_{j,j-1} by rows of row j (CSR over ocean unknowns)
+    int2 *d_lent = nullptr;              //   {local column in row j-1, slot-value index}
+    int *d_uptr = nullptr;               // A_{j-1,j} by columns in row j
+    int2 *d_uent = nullptr;              //   {local row in row j-1, slot-value index}
+    uint8_t *d_pin = nullptr;            // per cell: continuity row replaced by p = 0
+    RowFactors<double> f64;
+    RowFactors<float> f32;
+    int *d_ipiv = nullptr, *d_info = nullptr;
+    void *d_v = nullptr, *d_t = nullptr;  // row-layout work vectors (nocean, nmax) in the widest type
+    rocblas_handle blas = nullptr;
+};
+
+static inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+struct RowGeom {
+    const int *loc;
+    int n, m, l, periodic;
+};
+
+// ------------------------------------------------------------------ assembly
+template <typename T>
+__global__ void k_fill_t(T *__restrict__ a, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        a[q] = (T)0;
+}
+
+// D_j = A_jj: the slots of the cells of row j whose target lies in row j.
+template <typename T>
+__global__ void __launch_bounds__(256) k_row_block(Scalars s, RowGeom G, int j, int nrow, const double *__restrict__ J,
+                                                   const double *__restrict__ coef, double sign,
+                                                   const uint8_t *__restrict__ pin, T *__restrict__ D)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G.n * G.l) return;
+    const int i = t % G.n, k = t / G.n;
+    const int cell = i + G.n * (j + G.m * k);
+    const int lr = G.loc[cell];
+    if (lr < 0) return;
+    const size_t C = (size_t)s.ncell;
+    const bool introw = (6 * cell + S == s.rowintcon), pinned = pin[cell] != 0;
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; sl++) {
+        const int kk = SLOTS[sl].kk, e = SLOTS[sl].eq, v = SLOTS[sl].var;
+        if (kk_dj(kk) != 0) continue;
+        if ((introw && e == S) || (pinned && e == P)) continue;
+        const int nk = k + kk_dk(kk);
+        int ni = i + kk_di(kk);
+        if (G.periodic) ni = (ni + G.n) % G.n;
+        if (nk < 0 || nk >= G.l || ni < 0 || ni >= G.n) continue;
+        const int lc = G.loc[ni + G.n * (j + G.m * nk)];
+        if (lc < 0) continue;
+        D[(size_t)(lc + v) * nrow + lr + e] = (T)J[(size_t)sl * C + cell];
+    }
+    if (introw) {
+        const double d = sign * coef[6 * (size_t)cell + S];
+        D[(size_t)(lr + S) * nrow + lr + S] = (T)(d != 0.0 ? d : 1.0);
+    }
+    if (pinned) D[(size_t)(lr + P) * nrow + lr + P] = (T)1;
+}
+
+// X_j(:, c) = sum over the entries (r, a) of column c of A_{j-1,j}: a Minv_{j-1}(:, r)
+template <typename T>
+__global__ void __launch_bounds__(256) k_row_x(int nprev, const int *__restrict__ uptr, const int2 *__restrict__ uent,
+                                               const double *__restrict__ J, const T *__restrict__ Minv,
+                                               T *__restrict__ X)
+{
+    const int c = blockIdx.x;
+    const int q0 = uptr[c], q1 = uptr[c + 1];
+    T *out = X + (size_t)c * nprev;
+    for (int e = threadIdx.x; e < nprev; e += blockDim.x) {
+        T acc = 0;
+        for (int q = q0; q < q1; q++) {
+            const int2 en = uent[q];
+            acc += (T)J[en.y] * Minv[(size_t)en.x * nprev + e];
+        }
+        out[e] = acc;
+    }
+}
+
+// D_j(:, c) -= A_{j,j-1} X_j(:, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_row_schur(int nrow, int nprev, const int *__restrict__ lptr,
+                                                   const int2 *__restrict__ lent, const double *__restrict__ J,
+                                                   const T *__restrict__ X, T *__restrict__ D)
+{
+    const int c = blockIdx.x;
+    const T *xc = X + (size_t)c * nprev;
+    T *dc = D + (size_t)c * nrow;
+    for (int e = threadIdx.x; e < nrow; e += blockDim.x) {
+        T acc = 0;
+        for (int q = lptr[e]; q < lptr[e + 1]; q++) {
+            const int2 en = lent[q];
+            acc += (T)J[en.y] * xc[en.x];
+        }
+        dc[e] -= acc;
+    }
+}
+
+// ------------------------------------------------------------------ apply
+template <typename T>
+__global__ void k_row_gather(int nocean, const int *__restrict__ perm, const double *__restrict__ in, T *__restrict__ v)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nocean) v[u] = (T)in[perm[u]];
+}
+
+template <typename T>
+__global__ void k_row_scatter(int nocean, const int *__restrict__ perm, const T *__restrict__ v, double *__restrict__ out)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nocean) out[perm[u]] = (double)v[u];
+}
+
+__global__ void k_row_land(int ncell, const int *__restrict__ loc, const double *__restrict__ in, double *__restrict__ out)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncell || loc[cell] >= 0) return;
+#pragma unroll
+    for (int v = 0; v < 6; v++) out[6 * (size_t)cell + v] = in[6 * (size_t)cell + v];
+}
+
+// t = v_j - A_{j,j-1} z_{j-1}
+template <typename T>
+__global__ void k_row_lmv(int nrow, const int *__restrict__ lptr, const int2 *__restrict__ lent,
+                          const double *__restrict__ J, const T *__restrict__ vj, const T *__restrict__ zprev,
+                          T *__restrict__ t)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrow) return;
+    T acc = vj[e];
+    for (int q = lptr[e]; q < lptr[e + 1]; q++) {
+        const int2 en = lent[q];
+        acc -= (T)J[en.y] * zprev[en.x];
+    }
+    t[e] = acc;
+}
+
+// ------------------------------------------------------------------ host
+#define BLASCHK(expr)                                                                  \
+    do {                                                                               \
+        rocblas_status st_ = (expr);                                                   \
+        if (st_ != rocblas_status_success)                                             \
+            return set_error(IEMIC_EHIP, std::string(#expr) + ": rocBLAS/rocSOLVER status " + \
+                                             std::to_string((int)st_));                \
+    } while (0)
+
+void rowlu_free(iemic_ctx *c)
+{
+    RowLU *p = c->rowlu;
+    if (!p) return;
+    void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_uptr, p->d_uent, p->d_pin,
+                    p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (p->blas) rocblas_destroy_handle(p->blas);
+    delete p;
+    c->rowlu = nullptr;
+}
+
+static int rowlu_ensure(iemic_ctx *c)
+{
+    if (c->rowlu) return IEMIC_OK;
+    const int n = c->cfg.n, m = c->cfg.m, l = c->cfg.l;
+    const size_t C = (size_t)n * m * l;
+    if ((size_t)NSLOT * C >= (size_t)1 << 31)
+        return set_error(IEMIC_EUNSUPPORTED, "row preconditioner: grid too large for 32-bit slot indices");
+    RowLU *p = new RowLU();
+    c->rowlu = p;
+    p->m = m;
+    const std::vector<int> k0 = column_tops(c);
+    std::vector<int> rank((size_t)n * m);
+    for (int q = 0; q < n * m; q++) rank[q] = q;   // rows eliminated in j order
+    const std::vector<uint8_t> pin = pressure_pins(c, k0, rank);
+    for (uint8_t v : pin) p->npins += v;
+    // local numbering within rows and the ocean-unknown permutation
+    std::vector<int> loc(C, -1), perm;
+    p->nj.assign(m, 0);
+    p->ubase.assign(m + 1, 0);
+    for (int j = 0; j < m; j++) {
+        int base = 0;
+        for (int i = 0; i < n; i++)
+            for (int k = k0[i + n * j]; k < l; k++) {
+                const int cell = i + n * (j + m * k);
+                loc[cell] = base;
+                for (int v = 0; v < 6; v++) perm.push_back(6 * cell + v);
+                base += 6;
+            }
+        p->nj[j] = base;
+        p->ubase[j + 1] = p->ubase[j] + base;
+        p->nmax = std::max(p->nmax, base);
+    }
+    p->nocean = p->ubase[m];
+    p->doff.assign(m, 0);
+    p->xoff.assign(m, 0);
+    size_t tot = 0;
+    for (int j = 0; j < m; j++) {
+        p->doff[j] = tot;
+        tot += (size_t)p->nj[j] * p->nj[j];
+        if (j > 0) {
+            p->xoff[j] = tot;
+            tot += (size_t)p->nj[j - 1] * p->nj[j];
+        }
+    }
+    p->total = tot;
+    // couplings between neighbouring rows, by rows of A_{j,j-1} and by columns of A_{j-1,j}
+    const int ric = c->sc.rowintcon;
+    std::vector<std::vector<int2>> lrow(p->nocean), ucol(p->nocean);
+    for (int k = 0; k < l; k++)
+        for (int j = 0; j < m; j++)
+            for (int i = 0; i < n; i++) {
+                const int cell = i + n * (j + m * k);
+                if (loc[cell] < 0) continue;
+                const bool introw = (6 * cell + S == ric), pinned = pin[cell] != 0;
+                for (int sl = 0; sl < NSLOT; sl++) {
+                    const SlotDef &sd = SLOTS[sl];
+                    const int dj = kk_dj(sd.kk);
+                    if (dj == 0) continue;
+                    if ((introw && sd.eq == S) || (pinned && sd.eq == P)) continue;
+                    const int nj2 = j + dj, nk = k + kk_dk(sd.kk);
+                    int ni = i + kk_di(sd.kk);
+                    if (c->cfg.periodic) ni = (ni + n) % n;
+                    if (nj2 < 0 || nj2 >= m || nk < 0 || nk >= l || ni < 0 || ni >= n) continue;
+                    const int tcell = ni + n * (nj2 + m * nk);
+                    if (loc[tcell] < 0) continue;
+                    const int vidx = (int)((size_t)sl * C + cell);
+                    const int r_local = loc[cell] + sd.eq, c_local = loc[tcell] + sd.var;
+                    if (dj < 0)   // A_{j,j-1}: row in row j, column in row j-1
+                        lrow[p->ubase[j] + r_local].push_back(make_int2(c_local, vidx));
+                    else          // A_{j,j+1} = A_{(j+1)-1, j+1}: listed by its column in row j+1
+                        ucol[p->ubase[nj2] + c_local].push_back(make_int2(r_local, vidx));
+                }
+            }
+    // per-row CSR pointers are relative to the row (lptr_j[e] indexes the global entry array)
+    std::vector<int> lptr(p->nocean + 1, 0), uptr(p->nocean + 1, 0);
+    std::vector<int2> lent, uent;
+    for (int u = 0; u < p->nocean; u++) {
+        lptr[u] = (int)lent.size();
+        lent.insert(lent.end(), lrow[u].begin(), lrow[u].end());
+        uptr[u] = (int)uent.size();
+        uent.insert(uent.end(), ucol[u].begin(), ucol[u].end());
+    }
+    lptr[p->nocean] = (int)lent.size();
+    uptr[p->nocean] = (int)uent.size();
+    if (lent.empty()) lent.push_back(make_int2(0, 0));
+    if (uent.empty()) uent.push_back(make_int2(0, 0));
+    auto up = [&](auto **dst, const auto &v) -> int {
+        HIPCHK(hipMalloc((void **)dst, v.size() * sizeof(v[0])));
+        HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+        return IEMIC_OK;
+    };
+    int rc;
+    if ((rc = up(&p->d_loc, loc)) || (rc = up(&p->d_perm, perm)) || (rc = up(&p->d_lptr, lptr)) ||
+        (rc = up(&p->d_lent, lent)) || (rc = up(&p->d_uptr, uptr)) || (rc = up(&p->d_uent, uent)) ||
+        (rc = up(&p->d_pin, pin)))
+        return rc;
+    HIPCHK(hipMalloc(&p->d_ipiv, (size_t)std::max(p->nmax, 1) * sizeof(int)));
+    HIPCHK(hipMalloc(&p->d_info, (size_t)m * sizeof(int)));
+    HIPCHK(hipMalloc(&p->d_v, (size_t)std::max(p->nocean, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&p->d_t, (size_t)std::max(p->nmax, 1) * sizeof(double)));
+    BLASCHK(rocblas_create_handle(&p->blas));
+    BLASCHK(rocblas_set_stream(p->blas, c->stream));
+    // size the handle's workspace once for the largest row, so that the
+    // factorisation makes no allocation of its own per call
+    if (p->nmax > 0) {
+        size_t ws = 0;
+        double *dd = reinterpret_cast<double *>(p->d_v);
+        float *df = reinterpret_cast<float *>(p->d_v);
+        BLASCHK(rocblas_start_device_memory_size_query(p->blas));
+        rocsolver_dgetrf(p->blas, p->nmax, p->nmax, dd, p->nmax, p->d_ipiv, p->d_info);
+        rocsolver_dgetri(p->blas, p->nmax, dd, p->nmax, p->d_ipiv, p->d_info);
+        rocsolver_sgetrf(p->blas, p->nmax, p->nmax, df, p->nmax, p->d_ipiv, p->d_info);
+        rocsolver_sgetri(p->blas, p->nmax, df, p->nmax, p->d_ipiv, p->d_info);
+        BLASCHK(rocblas_stop_device_memory_size_query(p->blas, &ws));
+        BLASCHK(rocblas_set_device_memory_size(p->blas, ws));
+    }
+    return IEMIC_OK;
+}
+
+// typed rocSOLVER / rocBLAS entry points
+static rocblas_status getrf(rocblas_handle h, int n, double *A, int *ipiv, int *info)
+{
+    return rocsolver_dgetrf(h, n, n, A, n, ipiv, info);
+}
+static rocblas_status getrf(rocblas_handle h, int n, float *A, int *ipiv, int *info)
+{
+    return rocsolver_sgetrf(h, n, n, A, n, ipiv, info);
+}
+static rocblas_status getri(rocblas_handle h, int n, double *A, int *ipiv, int *info)
+{
+    return rocsolver_dgetri(h, n, A, n, ipiv, info);
+}
+static rocblas_status getri(rocblas_handle h, int n, float *A, int *ipiv, int *info)
+{
+    return rocsolver_sgetri(h, n, A, n, ipiv, info);
+}
+static rocblas_status gemv(rocblas_handle h, int rows, int cols, double alpha, const double *A, const double *x,
+                           double beta, double *y)
+{
+    return rocblas_dgemv(h, rocblas_operation_none, rows, cols, &alpha, A, rows, x, 1, &beta, y, 1);
+}
+static rocblas_status gemv(rocblas_handle h, int rows, int cols, float alpha, const float *A, const float *x,
+                           float beta, float *y)
+{
+    float a = alpha, b = beta;
+    return rocblas_sgemv(h, rocblas_operation_none, rows, cols, &a, A, rows, x, 1, &b, y, 1);
+}
+
+template <typename T>
+static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
+{
+    RowLU *p = c->rowlu;
+    if (!F.blk) HIPCHK(hipMalloc(&F.blk, std::max<size_t>(p->total, 1) * sizeof(T)));
+    const RowGeom G{p->d_loc, c->cfg.n, c->cfg.m, c->cfg.l, c->cfg.periodic};
+    BLASCHK(rocblas_set_pointer_mode(p->blas, rocblas_pointer_mode_host));
+    for (int j = 0; j < p->m; j++) {
+        const int nr = p->nj[j];
+        if (nr == 0) continue;
+        T *D = F.blk + p->doff[j];
+        hipLaunchKernelGGL(k_fill_t<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 4096)), dim3(256), 0, c->stream, D,
+                           (size_t)nr * nr);
+        hipLaunchKernelGGL(k_row_block<T>, dim3(nblk((size_t)c->cfg.n * c->cfg.l, 256)), dim3(256), 0, c->stream,
+                           c->sc, G, j, nr, c->d_J, c->d_coef, (double)c->cfg.int_sign, p->d_pin, D);
+        const int np = j > 0 ? p->nj[j - 1] : 0;
+        if (np > 0) {
+            T *X = F.blk + p->xoff[j];
+            const T *Mprev = F.blk + p->doff[j - 1];
+            hipLaunchKernelGGL(k_row_x<T>, dim3(nr), dim3(256), 0, c->stream, np, p->d_uptr + p->ubase[j], p->d_uent,
+                               c->d_J, Mprev, X);
+            hipLaunchKernelGGL(k_row_schur<T>, dim3(nr), dim3(256), 0, c->stream, nr, np, p->d_lptr + p->ubase[j],
+                               p->d_lent, c->d_J, X, D);
+        }
+        HIPCHK(hipGetLastError());
+        BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
+        BLASCHK(getri(p->blas, nr, D, p->d_ipiv, p->d_info + j));
+    }
+    std::vector<int> info(p->m, 0);
+    HIPCHK(hipMemcpyAsync(info.data(), p->d_info, p->m * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < p->m; j++)
+        if (p->nj[j] > 0 && info[j] != 0)
+            return set_error(IEMIC_ENOCONV, "row preconditioner: row block " + std::to_string(j) +
+                                                " is singular (pivot " + std::to_string(info[j]) + ")");
+    F.built = true;
+    F.version = c->jac_version;
+    return IEMIC_OK;
+}
+
+template <typename T>
+static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, double *out)
+{
+    RowLU *p = c->rowlu;
+    T *v = reinterpret_cast<T *>(p->d_v), *t = reinterpret_cast<T *>(p->d_t);
+    hipLaunchKernelGGL(k_row_land, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc.ncell, p->d_loc, in, out);
+    if (p->nocean == 0) return IEMIC_OK;
+    hipLaunchKernelGGL(k_row_gather<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm, in, v);
+    // forward: z_j = Minv_j (v_j - A_{j,j-1} z_{j-1}), z overwrites v
+    for (int j = 0; j < p->m; j++) {
+        const int nr = p->nj[j];
+        if (nr == 0) continue;
+        T *vj = v + p->ubase[j];
+        const int np = j > 0 ? p->nj[j - 1] : 0;
+        if (np > 0)
+            hipLaunchKernelGGL(k_row_lmv<T>, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, nr,
+                               p->d_lptr + p->ubase[j], p->d_lent, c->d_J, vj, v + p->ubase[j - 1], t);
+        else
+            HIPCHK(hipMemcpyAsync(t, vj, (size_t)nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        BLASCHK(gemv(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
+    }
+    // backward: z_{j-1} -= X_j z_j
+    for (int j = p->m - 1; j >= 1; j--) {
+        const int nr = p->nj[j], np = p->nj[j - 1];
+        if (nr == 0 || np == 0) continue;
+        BLASCHK(gemv(p->blas, np, nr, (T)-1, F.blk + p->xoff[j], v + p->ubase[j], (T)1, v + p->ubase[j - 1]));
+    }
+    hipLaunchKernelGGL(k_row_scatter<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm, v, out);
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+
+static bool row_current(const iemic_ctx *c, bool built, unsigned version)
+{
+    return built && version == c->jac_version;
+}
+
+int rowlu_build(iemic_ctx *c, int precision)
+{
+    int rc = rowlu_ensure(c);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    timed_begin_ext(c, precision ? 5 : 4);
+    rc = precision ? rowlu_factor<float>(c, p->f32) : rowlu_factor<double>(c, p->f64);
+    timed_end_ext(c, precision ? 5 : 4);
+    return rc;
+}
+
+int rowlu_ready(iemic_ctx *c, int precision)
+{
+    int rc = rowlu_ensure(c);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    const bool cur = precision ? row_current(c, p->f32.built, p->f32.version) : row_current(c, p->f64.built, p->f64.version);
+    return cur ? IEMIC_OK : rowlu_build(c, precision);
+}
+
+// M^{-1} in on the model's unknowns (FP64 in and out); the factors and the
+// sweeps run in FP64 (precision 0) or FP32 (precision 1).
+int rowlu_apply(iemic_ctx *c, int precision, const double *in, double *out)
+{
+    int rc = rowlu_ready(c, precision);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    timed_begin_ext(c, precision ? 6 : 3);
+    rc = precision ? rowlu_sweep<float>(c, p->f32, in, out) : rowlu_sweep<double>(c, p->f64, in, out);
+    timed_end_ext(c, precision ? 6 : 3);
+    return rc;
+}
+
+int rowlu_info(iemic_ctx *c, iemic_precond_info *info)
+{
+    int rc = rowlu_ensure(c);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    info->blocks = p->m;
+    info->max_block = p->nmax;
+    info->pins = p->npins;
+    info->levels = 2 * p->m - 1;
+    info->factor_bytes = (double)p->total * sizeof(double);
+    return IEMIC_OK;
+}
+
+}  // namespace iemic

@@ -9,10 +9,9 @@
 // Givens rotations (GMRESSolver.H:257-289).  The FP64 true residual
 // ||b - J x|| / ||b|| is checked at every restart and at the end.
 //
-// Mixed precision (precision = 1): an FP64 outer iterative refinement
-// r = b - J x (FP64 SpMV on the FP64 Jacobian), inner solve of J d = r with
-// FGMRES whose basis, SpMV (FP32 copy of J) and preconditioner run in FP32,
-// x += d, until the FP64 relative residual reaches tol.
+// Mixed precision (precision = 1): the preconditioner is factored and
+// applied in FP32 while the basis, the products with J and every residual
+// stay FP64, so the accuracy of the result is that of the FP64 iteration.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,10 +24,9 @@
 
 namespace iemic {
 
-int precond_build(iemic_ctx *c);
-int precond_apply(iemic_ctx *c, const double *d_in, double *d_out);
-int precond_apply_f32(iemic_ctx *c, const float *d_in, float *d_out);
-int precond_build_f32(iemic_ctx *c);
+int precond_ready(iemic_ctx *c, int precision);
+int precond_apply(iemic_ctx *c, int precision, const double *in, double *out, float *tmp32);
+int precond_check(iemic_ctx *c);
 
 template <typename T>
 struct Basis {
@@ -38,38 +36,52 @@ struct Basis {
 
 struct Krylov {
     Basis<double> b64;
-    Basis<float> b32;
-    double *d_h = nullptr;     // (restart+1) coefficients
-    double *d_part = nullptr;  // partial sums (blocks x (restart+1))
+    double *d_h = nullptr;     // Gram-Schmidt coefficients, two passes
+    double *d_part = nullptr;  // partial sums (blocks x (restart+2))
     int nblocks = 0;
-    double *d_r = nullptr, *d_d = nullptr;   // FP64 refinement residual / correction
-    float *J32 = nullptr;                    // FP32 copy of the slot values
-    float *coef32 = nullptr;
+    double *d_r = nullptr;     // FP64 residual
+    float *tmp32 = nullptr;    // 2 ndim: FP32 preconditioner input/output
     int hcap = 0;
+    int precision = 0;         // preconditioner precision of the current solve
 };
 
 // ------------------------------------------------------------------ kernels
+// part[q * gridDim.x + block] = sum over this block's rows of V[q] . w for
+// the NV vectors q = blockIdx.y * NV + (0..NV-1) below nv: one pass over the
+// rows reads each basis vector once and w once per group of NV.
+constexpr int NV = 8;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_multidot(const T *__restrict__ V, size_t ld, int nv,
                                                    const T *__restrict__ w, int n,
                                                    double *__restrict__ part)
 {
-    // part[q * gridDim.x + block] = sum over this block's rows of V[q] . w
-    __shared__ double sh[256];
-    int chunk = (n + gridDim.x - 1) / gridDim.x;
-    int r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
-    for (int q = 0; q < nv; q++) {
-        const T *v = V + (size_t)q * ld;
-        double acc = 0.0;
-        for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) acc += (double)v[r] * (double)w[r];
-        sh[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = sh[0];
-        __syncthreads();
+    __shared__ double sh[4][NV];
+    const int q0 = blockIdx.y * NV;
+    const int cnt = min(NV, nv - q0);
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const double wr = (double)w[r];
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            if (k < cnt) acc[k] += (double)V[(size_t)(q0 + k) * ld + r] * wr;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[k] = v;
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[wave][k] = acc[k];
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+        const int k = threadIdx.x;
+        part[(size_t)(q0 + k) * gridDim.x + blockIdx.x] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
     }
 }
 
@@ -115,67 +127,6 @@ __global__ void k_axpby(double *__restrict__ y, const double *__restrict__ x, do
     if (r < n) y[r] = a * x[r] + b * y[r];
 }
 
-template <typename A, typename B>
-__global__ void k_convert(A *__restrict__ dst, const B *__restrict__ src, size_t n)
-{
-    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) dst[r] = (A)src[r];
-}
-
-// FP32 SpMV on the slot values (same structure as the FP64 one)
-__global__ void __launch_bounds__(256) k_spmv32(Scalars s, const float *__restrict__ J,
-                                                const float *__restrict__ x, float *__restrict__ y)
-{
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= s.ncell) return;
-    const int i = cell % s.n + 1, j = (cell / s.n) % s.m + 1, k = cell / (s.n * s.m) + 1;
-    const size_t C = (size_t)s.ncell;
-    float acc[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int sl = 0; sl < NSLOT; sl++) {
-        int kk = SLOTS[sl].kk;
-        int ni = i + kk_di(kk), nj = j + kk_dj(kk), nk = k + kk_dk(kk);
-        if (s.periodic) ni = (ni < 1) ? ni + s.n : (ni > s.n ? ni - s.n : ni);
-        if (ni < 1 || ni > s.n || nj < 1 || nj > s.m || nk < 1 || nk > s.l) continue;
-        int nb = (ni - 1) + s.n * ((nj - 1) + s.m * (nk - 1));
-        acc[SLOTS[sl].eq] += J[(size_t)sl * C + cell] * x[6 * nb + SLOTS[sl].var];
-    }
-#pragma unroll
-    for (int e = 0; e < 6; e++) y[6 * (size_t)cell + e] = acc[e];
-}
-
-// integral row of the FP32 product (fixed-order two-stage reduction)
-__global__ void __launch_bounds__(256) k_dot32_partial(const float *__restrict__ a,
-                                                       const float *__restrict__ b, int n,
-                                                       double *__restrict__ part)
-{
-    __shared__ double sh[256];
-    double acc = 0.0;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-        acc += (double)a[q] * (double)b[q];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-__global__ void k_dot32_final(const double *__restrict__ part, int np, float *__restrict__ out,
-                              double scale, int slot)
-{
-    __shared__ double sh[256];
-    double acc = 0.0;
-    for (int q = threadIdx.x; q < np; q += blockDim.x) acc += part[q];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[slot] = (float)(scale * sh[0]);
-}
-
 // ------------------------------------------------------------------ host
 static inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
 
@@ -183,8 +134,7 @@ void krylov_free(iemic_ctx *c)
 {
     Krylov *k = c->kry;
     if (!k) return;
-    void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->b32.V, k->b32.Z, k->b32.w, k->d_h, k->d_part,
-                    k->d_r, k->d_d, k->J32, k->coef32};
+    void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->d_h, k->d_part, k->d_r, k->tmp32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete k;
@@ -214,34 +164,19 @@ static int ensure_krylov(iemic_ctx *c, int m)
         HIPCHK(hipMalloc(&k->d_part, (size_t)(m + 2) * k->nblocks * sizeof(double)));
         k->hcap = m;
     }
-    if (!k->d_r) {
-        HIPCHK(hipMalloc(&k->d_r, (size_t)c->sc.ndim * sizeof(double)));
-        HIPCHK(hipMalloc(&k->d_d, (size_t)c->sc.ndim * sizeof(double)));
-    }
+    if (!k->d_r) HIPCHK(hipMalloc(&k->d_r, (size_t)c->sc.ndim * sizeof(double)));
+    if (!k->tmp32) HIPCHK(hipMalloc(&k->tmp32, 2 * (size_t)c->sc.ndim * sizeof(float)));
     return IEMIC_OK;
 }
 
-// matrix and preconditioner in the working precision T
+// matrix and preconditioner of the iteration
 template <typename T> struct Ops;
 template <> struct Ops<double> {
     static int matvec(iemic_ctx *c, const double *x, double *y) { return launch_spmv(c, x, y); }
-    static int prec(iemic_ctx *c, const double *x, double *y) { return precond_apply(c, x, y); }
-};
-template <> struct Ops<float> {
-    static int matvec(iemic_ctx *c, const float *x, float *y)
+    static int prec(iemic_ctx *c, const double *x, double *y)
     {
-        Krylov *k = c->kry;
-        hipLaunchKernelGGL(k_spmv32, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc, k->J32, x, y);
-        if (c->sc.rowintcon >= 0) {
-            int nb = std::min(c->nred, nblk(c->sc.ndim, 256));
-            hipLaunchKernelGGL(k_dot32_partial, dim3(nb), dim3(256), 0, c->stream, k->coef32, x, c->sc.ndim, c->d_red);
-            hipLaunchKernelGGL(k_dot32_final, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, y,
-                               (double)c->cfg.int_sign, c->sc.rowintcon);
-        }
-        HIPCHK(hipGetLastError());
-        return IEMIC_OK;
+        return precond_apply(c, c->kry->precision, x, y, c->kry->tmp32);
     }
-    static int prec(iemic_ctx *c, const float *x, float *y) { return precond_apply_f32(c, x, y); }
 };
 
 // ||v||_2 via the multidot machinery (one vector against itself)
@@ -249,7 +184,7 @@ template <typename T>
 static int norm2(iemic_ctx *c, const T *v, int n, double &out)
 {
     Krylov *k = c->kry;
-    hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, v, (size_t)n, 1, v, n, k->d_part);
+    hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks, 1), dim3(256), 0, c->stream, v, (size_t)n, 1, v, n, k->d_part);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, k->d_part, k->nblocks, 1, k->d_h + 2 * (k->hcap + 2), 0);
     double r;
     HIPCHK(hipMemcpyAsync(&r, k->d_h + 2 * (k->hcap + 2), sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -270,11 +205,10 @@ __global__ void k_add_into(double *__restrict__ a, const double *__restrict__ b,
 // cycle (explicit residual check of GMRESSolver.H:229-248).
 template <typename T>
 static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, int verbose, int *its,
-                  double *est)
+                  int *cycles, double *est)
 {
     Krylov *k = c->kry;
-    Basis<T> &B = std::is_same<T, float>::value ? *reinterpret_cast<Basis<T> *>(&k->b32)
-                                                : *reinterpret_cast<Basis<T> *>(&k->b64);
+    Basis<T> &B = k->b64;
     const int n = c->sc.ndim;
     int rc = ensure_basis(B, m, n);
     if (rc) return rc;
@@ -284,6 +218,7 @@ static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, 
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(x, 0, (size_t)n * sizeof(T), c->stream));
     *its = 0;
+    *cycles = 0;
     *est = 1.0;
     if (bnorm == 0.0) { *est = 0.0; return IEMIC_OK; }
     std::vector<double> H((size_t)(m + 1) * m), cs(m + 1), sn(m + 1), s(m + 1), h(m + 2), y(m + 1);
@@ -303,14 +238,15 @@ static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, 
             if ((rc = Ops<T>::prec(c, vj, zj))) return rc;
             if ((rc = Ops<T>::matvec(c, zj, B.w))) return rc;
             // classical Gram-Schmidt, twice; coefficients summed on the device
-            hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
+            const dim3 gd(k->nblocks, (j + 1 + NV - 1) / NV);
+            hipLaunchKernelGGL(k_multidot<T>, gd, dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
             hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1, d_h, 0);
             hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(256), 0, c->stream, B.w, B.V, ld, j + 1, d_h, -1.0, n);
-            hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
+            hipLaunchKernelGGL(k_multidot<T>, gd, dim3(256), 0, c->stream, B.V, ld, j + 1, B.w, n, k->d_part);
             hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1, d_h2, 0);
             hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(256), 0, c->stream, B.w, B.V, ld, j + 1, d_h2, -1.0, n);
             hipLaunchKernelGGL(k_add_into, dim3(1), dim3(256), 0, c->stream, d_h, d_h2, j + 1);
-            hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks), dim3(256), 0, c->stream, B.w, (size_t)n, 1, B.w, n, k->d_part);
+            hipLaunchKernelGGL(k_multidot<T>, dim3(k->nblocks, 1), dim3(256), 0, c->stream, B.w, (size_t)n, 1, B.w, n, k->d_part);
             hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, k->d_part, k->nblocks, 1, d_h + j + 1, 0);
             HIPCHK(hipMemcpyAsync(h.data(), d_h, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
@@ -349,6 +285,7 @@ static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, 
         HIPCHK(hipMemcpyAsync(d_h, y.data(), j * sizeof(double), hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_update<T>, dim3(nb), dim3(256), 0, c->stream, x, B.Z, ld, j, d_h, 1.0, n);
         *its = total;
+        (*cycles)++;
         // explicit residual r = b - A x
         if ((rc = Ops<T>::matvec(c, x, B.w))) return rc;
         std::vector<double> one(1, 1.0);
@@ -381,71 +318,29 @@ int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, 
                  iemic_solve_info *info)
 {
     using clk = std::chrono::steady_clock;
-    int rc = ensure_krylov(c, std::max(p->restart, p->inner_restart));
+    if (p->restart < 1 || p->maxit < 1 || p->tol <= 0.0 || (p->precision != 0 && p->precision != 1))
+        return set_error(IEMIC_EINVAL, "solver parameters out of range");
+    int rc = ensure_krylov(c, p->restart);
     if (rc) return rc;
     Krylov *k = c->kry;
+    k->precision = p->precision;
     iemic_solve_info inf{};
     auto t0 = clk::now();
-    if (p->precision == 0) {
-        rc = precond_build(c);
-    } else {
-        if (!k->J32) {
-            HIPCHK(hipMalloc(&k->J32, (size_t)NSLOT * c->sc.ncell * sizeof(float)));
-            HIPCHK(hipMalloc(&k->coef32, (size_t)c->sc.ndim * sizeof(float)));
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(c->sc.ndim, 256)), dim3(256), 0, c->stream,
-                               k->coef32, c->d_coef, (size_t)c->sc.ndim);
-        }
-        size_t nj = (size_t)NSLOT * c->sc.ncell;
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(nj, 256)), dim3(256), 0, c->stream, k->J32, c->d_J, nj);
-        rc = precond_build_f32(c);
-    }
-    if (rc) return rc;
+    if ((rc = precond_ready(c, p->precision))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     auto t1 = clk::now();
     inf.t_precon_setup = std::chrono::duration<double>(t1 - t0).count();
     double bnorm;
     if ((rc = norm2(c, d_b, c->sc.ndim, bnorm))) return rc;
-    int n = c->sc.ndim;
-    if (p->precision == 0) {
-        int its;
-        double est;
-        rc = fgmres<double>(c, d_b, d_x, p->restart, p->tol, p->maxit, p->verbose, &its, &est);
-        if (rc) return rc;
-        inf.iterations = its;
-        if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &inf.rel_residual))) return rc;
-    } else {
-        // FP64 iterative refinement around an FP32 inner FGMRES
-        HIPCHK(hipMemsetAsync(d_x, 0, (size_t)n * sizeof(double), c->stream));
-        HIPCHK(hipMemcpyAsync(k->d_r, d_b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        double rel = 1.0;
-        int total = 0, steps = 0;
-        float *r32 = nullptr, *d32 = nullptr;
-        HIPCHK(hipMalloc(&r32, (size_t)n * sizeof(float)));
-        HIPCHK(hipMalloc(&d32, (size_t)n * sizeof(float)));
-        while (rel > p->tol && total < p->maxit && steps < 50) {
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n, 256)), dim3(256), 0, c->stream, r32, k->d_r,
-                               (size_t)n);
-            int its;
-            double est;
-            rc = fgmres<float>(c, r32, d32, p->inner_restart, p->inner_tol, p->maxit - total, p->verbose, &its, &est);
-            if (rc) break;
-            total += its;
-            steps++;
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n, 256)), dim3(256), 0, c->stream, k->d_d, d32,
-                               (size_t)n);
-            hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, d_x, k->d_d, 1.0, 1.0, n);
-            if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &rel))) break;
-            if (p->verbose) std::printf(" refinement %d: inner its %d, FP64 residual %.3e\n", steps, its, rel);
-        }
-        (void)hipFree(r32);
-        (void)hipFree(d32);
-        if (rc) return rc;
-        inf.iterations = total;
-        inf.refinements = steps;
-        inf.rel_residual = rel;
-    }
+    int its, cycles;
+    double est;
+    if ((rc = fgmres<double>(c, d_b, d_x, p->restart, p->tol, p->maxit, p->verbose, &its, &cycles, &est))) return rc;
+    inf.iterations = its;
+    inf.restarts = cycles;
+    if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &inf.rel_residual))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     inf.t_solve = std::chrono::duration<double>(clk::now() - t1).count();
+    if ((rc = precond_check(c))) return rc;
     inf.converged = inf.rel_residual <= p->tol;
     if (info) *info = inf;
     return inf.converged ? IEMIC_OK : set_error(IEMIC_ENOCONV, "solver did not reach the tolerance");

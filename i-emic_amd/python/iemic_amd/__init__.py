@@ -48,13 +48,20 @@ class Config(C.Structure):
 
 class SolverParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("restart", C.c_int), ("maxit", C.c_int),
-                ("precision", C.c_int), ("inner_restart", C.c_int), ("inner_tol", C.c_double),
-                ("verbose", C.c_int)]
+                ("precision", C.c_int), ("verbose", C.c_int)]
 
 
 class SolveInfo(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("rel_residual", C.c_double),
+    _fields_ = [("iterations", C.c_int), ("restarts", C.c_int), ("rel_residual", C.c_double),
                 ("t_precon_setup", C.c_double), ("t_solve", C.c_double), ("converged", C.c_int)]
+
+
+PREC_ROW_LU, PREC_COLUMN_ILU = 0, 1
+
+
+class PrecondInfo(C.Structure):
+    _fields_ = [("kind", C.c_int), ("blocks", C.c_int), ("max_block", C.c_int), ("pins", C.c_int),
+                ("levels", C.c_int), ("factor_bytes", C.c_double)]
 
 
 _P = C.POINTER
@@ -92,6 +99,8 @@ def load():
         "iemic_apply_matrix_dev": ([vp, vp, vp], C.c_int),
         "iemic_apply_matrix_transpose_dev": ([vp, vp, vp], C.c_int),
         "iemic_apply_mass": ([vp, dp, dp], C.c_int),
+        "iemic_set_preconditioner": ([vp, C.c_int], C.c_int),
+        "iemic_preconditioner_info": ([vp, _P(PrecondInfo)], C.c_int),
         "iemic_build_preconditioner": ([vp], C.c_int),
         "iemic_apply_precon": ([vp, dp, dp], C.c_int),
         "iemic_apply_precon_dev": ([vp, vp, vp], C.c_int),
@@ -279,6 +288,14 @@ class Ocean:
         out = np.empty(self.ndim)
         _check(self.lib.iemic_apply_mass(self.h, _d(v), _d(out)))
         return out
+
+    def setPreconditioner(self, kind):
+        _check(self.lib.iemic_set_preconditioner(self.h, int(kind)))
+
+    def preconditionerInfo(self):
+        info = PrecondInfo()
+        _check(self.lib.iemic_preconditioner_info(self.h, C.byref(info)))
+        return info
 
     def buildPreconditioner(self):
         _check(self.lib.iemic_build_preconditioner(self.h))

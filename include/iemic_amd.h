@@ -54,22 +54,37 @@ typedef struct {
 
 typedef struct iemic_ctx iemic_ctx;
 
+/* Preconditioners for the slot of Ocean::buildPreconditioner (Ocean.C:1377-1391),
+ * where the reference plugs in TRIOS::BlockPreconditioner. */
+enum {
+    IEMIC_PREC_ROW_LU = 0,      /* block-tridiagonal LU over zonal rows (default) */
+    IEMIC_PREC_COLUMN_ILU = 1   /* water-column block ILU(0)                      */
+};
+
+typedef struct {
+    int kind;             /* IEMIC_PREC_*                                           */
+    int blocks;           /* zonal rows / active water columns                      */
+    int max_block;        /* largest dense block (unknowns)                         */
+    int pins;             /* pressure basins (continuity rows replaced by p = 0)   */
+    int levels;           /* dependency levels of the sweeps                        */
+    double factor_bytes;  /* FP64 factor storage                                    */
+} iemic_precond_info;
+
 /* Solver controls (Ocean::getDefaultInitParameters, Ocean.C:2264-2301, and
  * GMRESSolver::setParameters, GMRESSolver.H:52-68). */
 typedef struct {
     double tol;          /* relative residual ||b - J x|| / ||b||            */
     int restart;         /* Krylov space size (Belos "Num Blocks")           */
     int maxit;           /* total iteration cap                              */
-    int precision;       /* 0: FP64 throughout; 1: FP32 inner solver with     */
-                         /*    FP64 iterative refinement of the outer residual */
-    int inner_restart;   /* mixed precision: inner FGMRES(m)                  */
-    double inner_tol;    /* mixed precision: inner relative tolerance         */
+    int precision;       /* 0: FP64 throughout; 1: preconditioner factors and  */
+                         /*    sweeps in FP32 inside the FP64 FGMRES (basis,   */
+                         /*    SpMV and residuals stay FP64)                   */
     int verbose;
 } iemic_solver_params;
 
 typedef struct {
-    int iterations;       /* total Krylov iterations (inner, in mixed mode)   */
-    int refinements;      /* outer FP64 refinement steps (mixed mode)         */
+    int iterations;       /* total FGMRES iterations                          */
+    int restarts;         /* FGMRES cycles                                    */
     double rel_residual;  /* true FP64 ||b - J x|| / ||b||                    */
     double t_precon_setup;/* seconds                                          */
     double t_solve;       /* seconds                                          */
@@ -123,6 +138,9 @@ int iemic_apply_matrix_transpose_dev(iemic_ctx *ctx, const double *d_v, double *
 int iemic_apply_mass(iemic_ctx *ctx, const double *v, double *out);
 
 /* ---- Ocean::buildPreconditioner / applyPrecon (Ocean.C:1377-1416) ------ */
+/* choose the preconditioner (IEMIC_PREC_*); drops existing factors */
+int iemic_set_preconditioner(iemic_ctx *ctx, int kind);
+int iemic_preconditioner_info(iemic_ctx *ctx, iemic_precond_info *info);
 int iemic_build_preconditioner(iemic_ctx *ctx);
 int iemic_apply_precon(iemic_ctx *ctx, const double *v, double *out);
 int iemic_apply_precon_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
@@ -142,7 +160,9 @@ int iemic_memcpy_d2h(void *dst, const void *src, long long bytes);
 int iemic_synchronize(iemic_ctx *ctx);
 
 /* ---- timing of the last device launches (HIP events, milliseconds) ----- */
-/* kernel: 0 = F assembly, 1 = J assembly, 2 = SpMV, 3 = preconditioner apply */
+/* kernel: 0 = F assembly, 1 = J assembly, 2 = SpMV, 3 = preconditioner
+ * apply (FP64), 4 = preconditioner build (FP64), 5 = build (FP32),
+ * 6 = apply (FP32) */
 int iemic_last_kernel_ms(const iemic_ctx *ctx, int kernel, double *ms);
 
 #ifdef __cplusplus
