@@ -55,8 +55,6 @@ struct Precond {
     int *d_rank = nullptr;         // ncol position in the elimination order
     uint8_t *d_pin = nullptr;      // ncell: 1 where the continuity row is replaced by p = 0
     int npins = 0;
-    double piv_thr = 1e-10, piv_rep = 1e-10, pstab = 0.0;
-    int pstabrel = 0;
     size_t *d_off = nullptr;       // ncol*9 block offsets (elements)
     int *d_order = nullptr;        // active columns sorted by level
     std::vector<int> lvl_off;      // nlevels+1 offsets into order
@@ -91,7 +89,7 @@ __global__ void k_fill(T *__restrict__ a, size_t n)
 // Scatter the slot values of every ocean cell into the column blocks.
 __global__ void __launch_bounds__(256) k_scatter_blocks(Scalars s, ColGeom G, const double *__restrict__ J,
                                                         const double *__restrict__ coef, double sign,
-                                                        double pstab, int pstabrel, const uint8_t *__restrict__ pin,
+                                                        const uint8_t *__restrict__ pin,
                                                         double *__restrict__ blk)
 {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,15 +115,9 @@ __global__ void __launch_bounds__(256) k_scatter_blocks(Scalars s, ColGeom G, co
         const int row = 6 * (k - k0r) + e, cc = 6 * (nk - G.k0[q]) + v;
         blk[G.off[(size_t)col * NB9 + sn] + (size_t)cc * br + row] = J[(size_t)sl * C + cell];
     }
-    if (pstab != 0.0 || pinned) {
+    if (pinned) {
         const int row = 6 * (k - k0r) + P;
-        double d = pstab;
-        if (pstabrel) {   // relative to the largest continuity-row entry of the cell
-            double mx = 0.0;
-            for (int sl = ROW0[P]; sl < ROW0[P + 1]; sl++) mx = fmax(mx, fabs(J[(size_t)sl * C + cell]));
-            d = pstab * mx;
-        }
-        blk[G.off[(size_t)col * NB9 + 4] + (size_t)row * br + row] += pinned ? 1.0 : d;
+        blk[G.off[(size_t)col * NB9 + 4] + (size_t)row * br + row] = 1.0;
     }
     if (introw) {
         const double d = sign * coef[6 * (size_t)cell + S];
@@ -615,18 +607,10 @@ static int ensure(iemic_ctx *c)
     c->prec = p;
     p->ncol = n * m;
     const std::vector<int> k0 = column_tops(c);
-    // elimination order: natural (i fastest) without the periodic wrap, or
-    // j fastest with the wrap couplings kept
-    const char *eo = std::getenv("IEMIC_ORDER");
-    const bool jfast = eo && std::atoi(eo) == 1 && c->cfg.periodic;
-    if (const char *v = std::getenv("IEMIC_PIVTHR")) p->piv_thr = std::atof(v);
-    p->piv_rep = p->piv_thr;
-    if (const char *v = std::getenv("IEMIC_PIVREP")) p->piv_rep = std::atof(v);
-    if (const char *v = std::getenv("IEMIC_PSTAB")) p->pstab = std::atof(v);
-    if (const char *v = std::getenv("IEMIC_PSTABREL")) p->pstabrel = std::atoi(v);
+    // elimination order: natural (i fastest), without the periodic wrap
     std::vector<int> rank(p->ncol), byrank(p->ncol);
     for (int j = 0; j < m; j++)
-        for (int i = 0; i < n; i++) rank[i + n * j] = jfast ? j + m * i : i + n * j;
+        for (int i = 0; i < n; i++) rank[i + n * j] = i + n * j;
     for (int r = 0; r < p->ncol; r++) byrank[rank[r]] = r;
     std::vector<int> nbr((size_t)p->ncol * NB9, -1), level(p->ncol, -1);
     std::vector<size_t> off((size_t)p->ncol * NB9, 0);
@@ -639,9 +623,7 @@ static int ensure(iemic_ctx *c)
         int lv = 0;
         for (int dj = -1; dj <= 1; dj++)
             for (int di = -1; di <= 1; di++) {
-                int ii = i + di;
-                const int jj = j + dj;
-                if (jfast) ii = (ii + n) % n;
+                const int ii = i + di, jj = j + dj;
                 if (ii < 0 || ii >= n || jj < 0 || jj >= m) continue;
                 const int q = ii + n * jj;
                 if (k0[q] >= l) continue;
@@ -702,11 +684,11 @@ static int factor64(iemic_ctx *c)
     const ColGeom G = geom(c);
     hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, c->stream, F.blk, p->total);
     hipLaunchKernelGGL(k_scatter_blocks, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc, G, c->d_J,
-                       c->d_coef, (double)c->cfg.int_sign, p->pstab, p->pstabrel, p->d_pin, F.blk);
+                       c->d_coef, (double)c->cfg.int_sign, p->d_pin, F.blk);
     for (int lv = 0; lv < p->nlevels; lv++) {
         const int w = p->lvl_off[lv + 1] - p->lvl_off[lv];
         hipLaunchKernelGGL(k_cbilu_level, dim3(w), dim3(256), 0, c->stream, p->d_order + p->lvl_off[lv], G, F.blk,
-                           p->d_scratch, p->bmax, p->d_piv, p->piv_thr, p->piv_rep);
+                           p->d_scratch, p->bmax, p->d_piv, 1e-10, 1e-10);
     }
     HIPCHK(hipGetLastError());
     F.built = true;

@@ -9,7 +9,8 @@
 // ocean unknowns is block tridiagonal in j (the periodic wrap in i stays
 // inside the row blocks) and factors exactly by block elimination:
 //     D_0 = A_00,   X_j = D_{j-1}^{-1} A_{j-1,j},   D_j = A_jj - A_{j,j-1} X_j,
-//     Minv_j = D_j^{-1}  (LU with partial pivoting, then the explicit inverse).
+//     Minv_j = D_j^{-1}  (LU with partial pivoting, then D_j Minv_j = I solved
+//                         with the LU factors).
 // The couplings A_{j,j-1}, A_{j-1,j} stay sparse (at most the stencil's
 // entries per unknown, read from the Jacobian slots through precomputed
 // index lists); Minv_j and X_j are dense, column-major.  Applying M^{-1} is a
@@ -64,6 +65,7 @@ struct RowLU {
     RowFactors<float> f32;
     int *d_ipiv = nullptr, *d_info = nullptr;
     void *d_v = nullptr, *d_t = nullptr;  // row-layout work vectors (nocean, nmax) in the widest type
+    void *d_w = nullptr;                  // nmax^2 scratch: right-hand sides of the inversion
     rocblas_handle blas = nullptr;
 };
 
@@ -80,6 +82,14 @@ __global__ void k_fill_t(T *__restrict__ a, size_t n)
 {
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         a[q] = (T)0;
+}
+
+template <typename T>
+__global__ void k_identity(T *__restrict__ W, int n)
+{
+    const size_t nn = (size_t)n * n;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += (size_t)gridDim.x * blockDim.x)
+        W[q] = (q % ((size_t)n + 1) == 0) ? (T)1 : (T)0;
 }
 
 // D_j = A_jj: the slots of the cells of row j whose target lies in row j.
@@ -207,7 +217,7 @@ void rowlu_free(iemic_ctx *c)
     RowLU *p = c->rowlu;
     if (!p) return;
     void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_uptr, p->d_uent, p->d_pin,
-                    p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t};
+                    p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t, p->d_w};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->blas) rocblas_destroy_handle(p->blas);
@@ -325,9 +335,9 @@ static int rowlu_ensure(iemic_ctx *c)
         float *df = reinterpret_cast<float *>(p->d_v);
         BLASCHK(rocblas_start_device_memory_size_query(p->blas));
         rocsolver_dgetrf(p->blas, p->nmax, p->nmax, dd, p->nmax, p->d_ipiv, p->d_info);
-        rocsolver_dgetri(p->blas, p->nmax, dd, p->nmax, p->d_ipiv, p->d_info);
+        rocsolver_dgetrs(p->blas, rocblas_operation_none, p->nmax, p->nmax, dd, p->nmax, p->d_ipiv, dd, p->nmax);
         rocsolver_sgetrf(p->blas, p->nmax, p->nmax, df, p->nmax, p->d_ipiv, p->d_info);
-        rocsolver_sgetri(p->blas, p->nmax, df, p->nmax, p->d_ipiv, p->d_info);
+        rocsolver_sgetrs(p->blas, rocblas_operation_none, p->nmax, p->nmax, df, p->nmax, p->d_ipiv, df, p->nmax);
         BLASCHK(rocblas_stop_device_memory_size_query(p->blas, &ws));
         BLASCHK(rocblas_set_device_memory_size(p->blas, ws));
     }
@@ -343,13 +353,15 @@ static rocblas_status getrf(rocblas_handle h, int n, float *A, int *ipiv, int *i
 {
     return rocsolver_sgetrf(h, n, n, A, n, ipiv, info);
 }
-static rocblas_status getri(rocblas_handle h, int n, double *A, int *ipiv, int *info)
+// B = A^{-1} B from the LU factors of A (two triangular solves with n
+// right-hand sides, BLAS-3 shaped)
+static rocblas_status getrs(rocblas_handle h, int n, const double *A, const int *ipiv, double *B)
 {
-    return rocsolver_dgetri(h, n, A, n, ipiv, info);
+    return rocsolver_dgetrs(h, rocblas_operation_none, n, n, const_cast<double *>(A), n, ipiv, B, n);
 }
-static rocblas_status getri(rocblas_handle h, int n, float *A, int *ipiv, int *info)
+static rocblas_status getrs(rocblas_handle h, int n, const float *A, const int *ipiv, float *B)
 {
-    return rocsolver_sgetri(h, n, A, n, ipiv, info);
+    return rocsolver_sgetrs(h, rocblas_operation_none, n, n, const_cast<float *>(A), n, ipiv, B, n);
 }
 static rocblas_status gemv(rocblas_handle h, int rows, int cols, double alpha, const double *A, const double *x,
                            double beta, double *y)
@@ -368,6 +380,7 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
 {
     RowLU *p = c->rowlu;
     if (!F.blk) HIPCHK(hipMalloc(&F.blk, std::max<size_t>(p->total, 1) * sizeof(T)));
+    if (!p->d_w) HIPCHK(hipMalloc(&p->d_w, std::max<size_t>((size_t)p->nmax * p->nmax, 1) * sizeof(double)));
     const RowGeom G{p->d_loc, c->cfg.n, c->cfg.m, c->cfg.l, c->cfg.periodic};
     BLASCHK(rocblas_set_pointer_mode(p->blas, rocblas_pointer_mode_host));
     for (int j = 0; j < p->m; j++) {
@@ -389,7 +402,12 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
         }
         HIPCHK(hipGetLastError());
         BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
-        BLASCHK(getri(p->blas, nr, D, p->d_ipiv, p->d_info + j));
+        // Minv_j: solve D W = I, then W replaces the factors
+        T *W = reinterpret_cast<T *>(p->d_w);
+        hipLaunchKernelGGL(k_identity<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 8192)), dim3(256), 0, c->stream, W,
+                           nr);
+        BLASCHK(getrs(p->blas, nr, D, p->d_ipiv, W));
+        HIPCHK(hipMemcpyAsync(D, W, (size_t)nr * nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     }
     std::vector<int> info(p->m, 0);
     HIPCHK(hipMemcpyAsync(info.data(), p->d_info, p->m * sizeof(int), hipMemcpyDeviceToHost, c->stream));

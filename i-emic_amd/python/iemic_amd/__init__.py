@@ -325,6 +325,27 @@ class Ocean:
             _check(rc)
         return x, info
 
+    # ---- device-resident variants (DeviceArray arguments)
+    def computeRHS_dev(self, dx, dF):
+        _check(self.lib.iemic_rhs_dev(self.h, dx.ptr, dF.ptr))
+
+    def computeJacobian_dev(self, dx):
+        _check(self.lib.iemic_jacobian_dev(self.h, dx.ptr))
+
+    def applyMatrix_dev(self, dv, dout):
+        _check(self.lib.iemic_apply_matrix_dev(self.h, dv.ptr, dout.ptr))
+
+    def solve_dev(self, db, dx, params=None, raise_on_noconv=True):
+        info = SolveInfo()
+        p = params if params is not None else self.solver_params()
+        rc = self.lib.iemic_solve_dev(self.h, C.byref(p), db.ptr, dx.ptr, C.byref(info))
+        if rc != 0 and (raise_on_noconv or rc != -6):
+            _check(rc)
+        return info
+
+    def synchronize(self):
+        _check(self.lib.iemic_synchronize(self.h))
+
     def kernel_ms(self, which):
         v = C.c_double()
         _check(self.lib.iemic_last_kernel_ms(self.h, which, C.byref(v)))
