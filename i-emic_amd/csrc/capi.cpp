@@ -314,6 +314,27 @@ int iemic_apply_precon(iemic_ctx *c, const double *v, double *out)
     return precond_check(c);
 }
 
+int iemic_apply_precon_fp32(iemic_ctx *c, const double *v, double *out)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    float *tmp = nullptr;
+    if (c->prec_kind != IEMIC_PREC_ROW_LU)
+        HIPCHK(hipMalloc(&tmp, 2 * (size_t)c->sc.ndim * sizeof(float)));
+    size_t bytes = (size_t)c->sc.ndim * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_x, v, bytes, hipMemcpyHostToDevice, c->stream));
+    rc = precond_apply(c, 1, c->d_x, c->d_y, tmp);
+    if (rc == IEMIC_OK) {
+        HIPCHK(hipMemcpyAsync(out, c->d_y, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        rc = precond_check(c);
+    }
+    if (tmp) (void)hipFree(tmp);
+    return rc;
+}
+
 void iemic_solver_defaults(iemic_solver_params *p)
 {
     if (!p) return;

@@ -578,23 +578,23 @@ std::vector<uint8_t> pressure_pins(const iemic_ctx *c, const std::vector<int> &k
                     unite(cell_of(i1, j, k), cell_of(i, j + 1, k));
                 }
             }
+    // per basin: the cell of the highest-ranked column, then the highest level
     std::vector<long long> best(C, -1);
-    std::vector<int> byrank(ncol);
-    for (int q = 0; q < ncol; q++) byrank[rank[q]] = q;
+    std::vector<int> bestcell(C, -1);
     for (int k = 0; k < l; k++)
         for (int j = 0; j < m; j++)
             for (int i = 0; i < n; i++) {
                 if (!wet(i, j, k)) continue;
-                const int root = find(cell_of(i, j, k));
+                const int cl = cell_of(i, j, k), root = find(cl);
                 const long long key = (long long)rank[i + n * j] * l + k;
-                long long &b = best[(size_t)root];
-                if (b < 0 || key > b) b = key;
+                if (best[root] < 0 || key > best[root]) {
+                    best[root] = key;
+                    bestcell[root] = cl;
+                }
             }
-    for (size_t q = 0; q < C; q++) {
-        if (best[q] < 0) continue;
-        const int colr = (int)(best[q] / l), k = (int)(best[q] % l);
-        pinv[(size_t)k * ncol + byrank[colr]] = 1;
-    }
+    for (size_t q = 0; q < C; q++)
+        if (bestcell[q] >= 0) pinv[(size_t)bestcell[q]] = 1;
+    (void)ncol;
     return pinv;
 }
 

@@ -23,6 +23,9 @@
 // one continuity row per pressure basin is replaced by p = 0 (pressure_pins),
 // which removes the pressure null vectors of J from the factors.
 //
+// The factored matrix is equilibrated first, R M C with power-of-two row and
+// column scales (exact in floating point), and M^{-1} = C (R M C)^{-1} R.
+//
 // Dense LU and inversion are rocSOLVER calls, the matrix-vector products
 // rocBLAS GEMV; the block assembly, the sparse Schur-complement updates and
 // the row gathers are kernels of this file.  The FP32 factors are computed
@@ -61,6 +64,9 @@ struct RowLU {
     int *d_uptr = nullptr;               // A_{j-1,j} by columns in row j
     int2 *d_uent = nullptr;              //   {local row in row j-1, slot-value index}
     uint8_t *d_pin = nullptr;            // per cell: continuity row replaced by p = 0
+    double *d_rs = nullptr, *d_cs = nullptr;   // ndim row / column scales (powers of two)
+    double *d_lval = nullptr, *d_uval = nullptr;  // scaled coupling values of lent / uent
+    int nl = 0, nu = 0;
     RowFactors<double> f64;
     RowFactors<float> f32;
     int *d_ipiv = nullptr, *d_info = nullptr;
@@ -92,11 +98,107 @@ __global__ void k_identity(T *__restrict__ W, int n)
         W[q] = (q % ((size_t)n + 1) == 0) ? (T)1 : (T)0;
 }
 
+// ------------------------------------------------------------------ scaling
+// The factored matrix is R M C with diagonal R, C of powers of two (exact in
+// floating point): R equilibrates the rows, C the columns of R M.  This keeps
+// the FP32 factors in range and well scaled; M^{-1} = C (R M C)^{-1} R.
+__device__ __forceinline__ double pow2_inv(double mx) { return mx > 0.0 ? ldexp(1.0, -ilogb(mx)) : 1.0; }
+
+// target cell of slot sl of cell (i,j,k) in the row numbering, -1 if none
+__device__ __forceinline__ int slot_target(const RowGeom &G, int i, int j, int k, int kk)
+{
+    int ni = i + kk_di(kk);
+    const int nj = j + kk_dj(kk), nk = k + kk_dk(kk);
+    if (G.periodic) ni = (ni + G.n) % G.n;
+    if (ni < 0 || ni >= G.n || nj < 0 || nj >= G.m || nk < 0 || nk >= G.l) return -1;
+    const int t = ni + G.n * (nj + G.m * nk);
+    return G.loc[t] >= 0 ? t : -1;
+}
+
+__global__ void __launch_bounds__(256) k_row_scale(Scalars s, RowGeom G, const double *__restrict__ J,
+                                                   const double *__restrict__ coef, double sign,
+                                                   const uint8_t *__restrict__ pin, double *__restrict__ rs)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= s.ncell) return;
+    if (G.loc[cell] < 0) {
+        for (int e = 0; e < 6; e++) rs[6 * (size_t)cell + e] = 1.0;
+        return;
+    }
+    const int i = cell % G.n, j = (cell / G.n) % G.m, k = cell / (G.n * G.m);
+    const size_t C = (size_t)s.ncell;
+    double mx[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; sl++)
+        if (slot_target(G, i, j, k, SLOTS[sl].kk) >= 0)
+            mx[SLOTS[sl].eq] = fmax(mx[SLOTS[sl].eq], fabs(J[(size_t)sl * C + cell]));
+    if (pin[cell]) mx[P] = 1.0;
+    if (6 * cell + S == s.rowintcon) {
+        const double d = sign * coef[6 * (size_t)cell + S];
+        mx[S] = d != 0.0 ? fabs(d) : 1.0;
+    }
+    for (int e = 0; e < 6; e++) rs[6 * (size_t)cell + e] = pow2_inv(mx[e]);
+}
+
+// column scales of R M: each cell gathers the entries of its six columns
+// from the cells whose slots reference it
+__global__ void __launch_bounds__(256) k_col_scale(Scalars s, RowGeom G, const double *__restrict__ J,
+                                                   const double *__restrict__ coef, double sign,
+                                                   const uint8_t *__restrict__ pin, const double *__restrict__ rs,
+                                                   double *__restrict__ cs)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= s.ncell) return;
+    if (G.loc[cell] < 0) {
+        for (int e = 0; e < 6; e++) cs[6 * (size_t)cell + e] = 1.0;
+        return;
+    }
+    const int i = cell % G.n, j = (cell / G.n) % G.m, k = cell / (G.n * G.m);
+    const size_t C = (size_t)s.ncell;
+    double mx[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; sl++) {
+        const int kk = SLOTS[sl].kk;
+        // the owner of this entry sits at -offset(kk) from this column cell
+        int oi = i - kk_di(kk);
+        const int oj = j - kk_dj(kk), ok = k - kk_dk(kk);
+        if (G.periodic) oi = (oi + G.n) % G.n;
+        if (oi < 0 || oi >= G.n || oj < 0 || oj >= G.m || ok < 0 || ok >= G.l) continue;
+        const int owner = oi + G.n * (oj + G.m * ok);
+        if (G.loc[owner] < 0) continue;
+        const int e = SLOTS[sl].eq;
+        if ((pin[owner] && e == P) || (6 * owner + S == s.rowintcon && e == S)) continue;
+        mx[SLOTS[sl].var] = fmax(mx[SLOTS[sl].var], fabs(J[(size_t)sl * C + owner]) * rs[6 * (size_t)owner + e]);
+    }
+    if (pin[cell]) mx[P] = fmax(mx[P], rs[6 * (size_t)cell + P]);
+    if (6 * cell + S == s.rowintcon) {
+        const double d = sign * coef[6 * (size_t)cell + S];
+        mx[S] = fmax(mx[S], (d != 0.0 ? fabs(d) : 1.0) * rs[6 * (size_t)cell + S]);
+    }
+    for (int e = 0; e < 6; e++) cs[6 * (size_t)cell + e] = pow2_inv(mx[e]);
+}
+
+// scaled values of coupling entries {other-row local index, slot index}:
+// val = J * rs(row) * cs(col); row = the entry's own unknown, col its target
+__global__ void k_scale_entries(int nent, const int2 *__restrict__ ent, Scalars s, RowGeom G,
+                                const double *__restrict__ J, const double *__restrict__ rs,
+                                const double *__restrict__ cs, double *__restrict__ val)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nent) return;
+    const int vidx = ent[q].y;
+    const int sl = vidx / s.ncell, cell = vidx % s.ncell;
+    const int i = cell % G.n, j = (cell / G.n) % G.m, k = cell / (G.n * G.m);
+    const int t = slot_target(G, i, j, k, SLOTS[sl].kk);
+    val[q] = t < 0 ? 0.0 : J[vidx] * rs[6 * (size_t)cell + SLOTS[sl].eq] * cs[6 * (size_t)t + SLOTS[sl].var];
+}
+
 // D_j = A_jj: the slots of the cells of row j whose target lies in row j.
 template <typename T>
 __global__ void __launch_bounds__(256) k_row_block(Scalars s, RowGeom G, int j, int nrow, const double *__restrict__ J,
                                                    const double *__restrict__ coef, double sign,
-                                                   const uint8_t *__restrict__ pin, T *__restrict__ D)
+                                                   const uint8_t *__restrict__ pin, const double *__restrict__ rs,
+                                                   const double *__restrict__ cs, T *__restrict__ D)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= G.n * G.l) return;
@@ -115,21 +217,24 @@ __global__ void __launch_bounds__(256) k_row_block(Scalars s, RowGeom G, int j, 
         int ni = i + kk_di(kk);
         if (G.periodic) ni = (ni + G.n) % G.n;
         if (nk < 0 || nk >= G.l || ni < 0 || ni >= G.n) continue;
-        const int lc = G.loc[ni + G.n * (j + G.m * nk)];
+        const int tc = ni + G.n * (j + G.m * nk);
+        const int lc = G.loc[tc];
         if (lc < 0) continue;
-        D[(size_t)(lc + v) * nrow + lr + e] = (T)J[(size_t)sl * C + cell];
+        D[(size_t)(lc + v) * nrow + lr + e] =
+            (T)(J[(size_t)sl * C + cell] * rs[6 * (size_t)cell + e] * cs[6 * (size_t)tc + v]);
     }
     if (introw) {
         const double d = sign * coef[6 * (size_t)cell + S];
-        D[(size_t)(lr + S) * nrow + lr + S] = (T)(d != 0.0 ? d : 1.0);
+        D[(size_t)(lr + S) * nrow + lr + S] =
+            (T)((d != 0.0 ? d : 1.0) * rs[6 * (size_t)cell + S] * cs[6 * (size_t)cell + S]);
     }
-    if (pinned) D[(size_t)(lr + P) * nrow + lr + P] = (T)1;
+    if (pinned) D[(size_t)(lr + P) * nrow + lr + P] = (T)(rs[6 * (size_t)cell + P] * cs[6 * (size_t)cell + P]);
 }
 
 // X_j(:, c) = sum over the entries (r, a) of column c of A_{j-1,j}: a Minv_{j-1}(:, r)
 template <typename T>
 __global__ void __launch_bounds__(256) k_row_x(int nprev, const int *__restrict__ uptr, const int2 *__restrict__ uent,
-                                               const double *__restrict__ J, const T *__restrict__ Minv,
+                                               const double *__restrict__ uval, const T *__restrict__ Minv,
                                                T *__restrict__ X)
 {
     const int c = blockIdx.x;
@@ -138,8 +243,7 @@ __global__ void __launch_bounds__(256) k_row_x(int nprev, const int *__restrict_
     for (int e = threadIdx.x; e < nprev; e += blockDim.x) {
         T acc = 0;
         for (int q = q0; q < q1; q++) {
-            const int2 en = uent[q];
-            acc += (T)J[en.y] * Minv[(size_t)en.x * nprev + e];
+            acc += (T)uval[q] * Minv[(size_t)uent[q].x * nprev + e];
         }
         out[e] = acc;
     }
@@ -148,7 +252,7 @@ __global__ void __launch_bounds__(256) k_row_x(int nprev, const int *__restrict_
 // D_j(:, c) -= A_{j,j-1} X_j(:, c)
 template <typename T>
 __global__ void __launch_bounds__(256) k_row_schur(int nrow, int nprev, const int *__restrict__ lptr,
-                                                   const int2 *__restrict__ lent, const double *__restrict__ J,
+                                                   const int2 *__restrict__ lent, const double *__restrict__ lval,
                                                    const T *__restrict__ X, T *__restrict__ D)
 {
     const int c = blockIdx.x;
@@ -156,27 +260,26 @@ __global__ void __launch_bounds__(256) k_row_schur(int nrow, int nprev, const in
     T *dc = D + (size_t)c * nrow;
     for (int e = threadIdx.x; e < nrow; e += blockDim.x) {
         T acc = 0;
-        for (int q = lptr[e]; q < lptr[e + 1]; q++) {
-            const int2 en = lent[q];
-            acc += (T)J[en.y] * xc[en.x];
-        }
+        for (int q = lptr[e]; q < lptr[e + 1]; q++) acc += (T)lval[q] * xc[lent[q].x];
         dc[e] -= acc;
     }
 }
 
 // ------------------------------------------------------------------ apply
 template <typename T>
-__global__ void k_row_gather(int nocean, const int *__restrict__ perm, const double *__restrict__ in, T *__restrict__ v)
+__global__ void k_row_gather(int nocean, const int *__restrict__ perm, const double *__restrict__ rs,
+                             const double *__restrict__ in, T *__restrict__ v)
 {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < nocean) v[u] = (T)in[perm[u]];
+    if (u < nocean) v[u] = (T)(rs[perm[u]] * in[perm[u]]);
 }
 
 template <typename T>
-__global__ void k_row_scatter(int nocean, const int *__restrict__ perm, const T *__restrict__ v, double *__restrict__ out)
+__global__ void k_row_scatter(int nocean, const int *__restrict__ perm, const double *__restrict__ cs,
+                              const T *__restrict__ v, double *__restrict__ out)
 {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < nocean) out[perm[u]] = (double)v[u];
+    if (u < nocean) out[perm[u]] = cs[perm[u]] * (double)v[u];
 }
 
 __global__ void k_row_land(int ncell, const int *__restrict__ loc, const double *__restrict__ in, double *__restrict__ out)
@@ -190,16 +293,13 @@ __global__ void k_row_land(int ncell, const int *__restrict__ loc, const double 
 // t = v_j - A_{j,j-1} z_{j-1}
 template <typename T>
 __global__ void k_row_lmv(int nrow, const int *__restrict__ lptr, const int2 *__restrict__ lent,
-                          const double *__restrict__ J, const T *__restrict__ vj, const T *__restrict__ zprev,
+                          const double *__restrict__ lval, const T *__restrict__ vj, const T *__restrict__ zprev,
                           T *__restrict__ t)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nrow) return;
     T acc = vj[e];
-    for (int q = lptr[e]; q < lptr[e + 1]; q++) {
-        const int2 en = lent[q];
-        acc -= (T)J[en.y] * zprev[en.x];
-    }
+    for (int q = lptr[e]; q < lptr[e + 1]; q++) acc -= (T)lval[q] * zprev[lent[q].x];
     t[e] = acc;
 }
 
@@ -217,6 +317,7 @@ void rowlu_free(iemic_ctx *c)
     RowLU *p = c->rowlu;
     if (!p) return;
     void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_uptr, p->d_uent, p->d_pin,
+                    p->d_rs, p->d_cs, p->d_lval, p->d_uval,
                     p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t, p->d_w};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -321,6 +422,12 @@ static int rowlu_ensure(iemic_ctx *c)
         (rc = up(&p->d_lent, lent)) || (rc = up(&p->d_uptr, uptr)) || (rc = up(&p->d_uent, uent)) ||
         (rc = up(&p->d_pin, pin)))
         return rc;
+    p->nl = (int)lent.size();
+    p->nu = (int)uent.size();
+    HIPCHK(hipMalloc(&p->d_rs, (size_t)c->sc.ndim * sizeof(double)));
+    HIPCHK(hipMalloc(&p->d_cs, (size_t)c->sc.ndim * sizeof(double)));
+    HIPCHK(hipMalloc(&p->d_lval, (size_t)p->nl * sizeof(double)));
+    HIPCHK(hipMalloc(&p->d_uval, (size_t)p->nu * sizeof(double)));
     HIPCHK(hipMalloc(&p->d_ipiv, (size_t)std::max(p->nmax, 1) * sizeof(int)));
     HIPCHK(hipMalloc(&p->d_info, (size_t)m * sizeof(int)));
     HIPCHK(hipMalloc(&p->d_v, (size_t)std::max(p->nocean, 1) * sizeof(double)));
@@ -383,6 +490,16 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
     if (!p->d_w) HIPCHK(hipMalloc(&p->d_w, std::max<size_t>((size_t)p->nmax * p->nmax, 1) * sizeof(double)));
     const RowGeom G{p->d_loc, c->cfg.n, c->cfg.m, c->cfg.l, c->cfg.periodic};
     BLASCHK(rocblas_set_pointer_mode(p->blas, rocblas_pointer_mode_host));
+    const double sign = (double)c->cfg.int_sign;
+    const int nbc = nblk(c->sc.ncell, 256);
+    hipLaunchKernelGGL(k_row_scale, dim3(nbc), dim3(256), 0, c->stream, c->sc, G, c->d_J, c->d_coef, sign, p->d_pin,
+                       p->d_rs);
+    hipLaunchKernelGGL(k_col_scale, dim3(nbc), dim3(256), 0, c->stream, c->sc, G, c->d_J, c->d_coef, sign, p->d_pin,
+                       p->d_rs, p->d_cs);
+    hipLaunchKernelGGL(k_scale_entries, dim3(nblk(p->nl, 256)), dim3(256), 0, c->stream, p->nl, p->d_lent, c->sc, G,
+                       c->d_J, p->d_rs, p->d_cs, p->d_lval);
+    hipLaunchKernelGGL(k_scale_entries, dim3(nblk(p->nu, 256)), dim3(256), 0, c->stream, p->nu, p->d_uent, c->sc, G,
+                       c->d_J, p->d_rs, p->d_cs, p->d_uval);
     for (int j = 0; j < p->m; j++) {
         const int nr = p->nj[j];
         if (nr == 0) continue;
@@ -390,15 +507,15 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
         hipLaunchKernelGGL(k_fill_t<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 4096)), dim3(256), 0, c->stream, D,
                            (size_t)nr * nr);
         hipLaunchKernelGGL(k_row_block<T>, dim3(nblk((size_t)c->cfg.n * c->cfg.l, 256)), dim3(256), 0, c->stream,
-                           c->sc, G, j, nr, c->d_J, c->d_coef, (double)c->cfg.int_sign, p->d_pin, D);
+                           c->sc, G, j, nr, c->d_J, c->d_coef, sign, p->d_pin, p->d_rs, p->d_cs, D);
         const int np = j > 0 ? p->nj[j - 1] : 0;
         if (np > 0) {
             T *X = F.blk + p->xoff[j];
             const T *Mprev = F.blk + p->doff[j - 1];
             hipLaunchKernelGGL(k_row_x<T>, dim3(nr), dim3(256), 0, c->stream, np, p->d_uptr + p->ubase[j], p->d_uent,
-                               c->d_J, Mprev, X);
+                               p->d_uval, Mprev, X);
             hipLaunchKernelGGL(k_row_schur<T>, dim3(nr), dim3(256), 0, c->stream, nr, np, p->d_lptr + p->ubase[j],
-                               p->d_lent, c->d_J, X, D);
+                               p->d_lent, p->d_lval, X, D);
         }
         HIPCHK(hipGetLastError());
         BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
@@ -428,7 +545,8 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
     T *v = reinterpret_cast<T *>(p->d_v), *t = reinterpret_cast<T *>(p->d_t);
     hipLaunchKernelGGL(k_row_land, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc.ncell, p->d_loc, in, out);
     if (p->nocean == 0) return IEMIC_OK;
-    hipLaunchKernelGGL(k_row_gather<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm, in, v);
+    hipLaunchKernelGGL(k_row_gather<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
+                       p->d_rs, in, v);
     // forward: z_j = Minv_j (v_j - A_{j,j-1} z_{j-1}), z overwrites v
     for (int j = 0; j < p->m; j++) {
         const int nr = p->nj[j];
@@ -437,7 +555,7 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
         const int np = j > 0 ? p->nj[j - 1] : 0;
         if (np > 0)
             hipLaunchKernelGGL(k_row_lmv<T>, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, nr,
-                               p->d_lptr + p->ubase[j], p->d_lent, c->d_J, vj, v + p->ubase[j - 1], t);
+                               p->d_lptr + p->ubase[j], p->d_lent, p->d_lval, vj, v + p->ubase[j - 1], t);
         else
             HIPCHK(hipMemcpyAsync(t, vj, (size_t)nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
         BLASCHK(gemv(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
@@ -448,7 +566,8 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
         if (nr == 0 || np == 0) continue;
         BLASCHK(gemv(p->blas, np, nr, (T)-1, F.blk + p->xoff[j], v + p->ubase[j], (T)1, v + p->ubase[j - 1]));
     }
-    hipLaunchKernelGGL(k_row_scatter<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm, v, out);
+    hipLaunchKernelGGL(k_row_scatter<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
+                       p->d_cs, v, out);
     HIPCHK(hipGetLastError());
     return IEMIC_OK;
 }

@@ -104,6 +104,7 @@ def load():
         "iemic_build_preconditioner": ([vp], C.c_int),
         "iemic_apply_precon": ([vp, dp, dp], C.c_int),
         "iemic_apply_precon_dev": ([vp, vp, vp], C.c_int),
+        "iemic_apply_precon_fp32": ([vp, dp, dp], C.c_int),
         "iemic_solver_defaults": ([_P(SolverParams)], None),
         "iemic_solve": ([vp, _P(SolverParams), dp, dp, _P(SolveInfo)], C.c_int),
         "iemic_solve_dev": ([vp, _P(SolverParams), vp, vp, _P(SolveInfo)], C.c_int),
@@ -304,6 +305,12 @@ class Ocean:
         v = _f64(v)
         out = np.empty(self.ndim)
         _check(self.lib.iemic_apply_precon(self.h, _d(v), _d(out)))
+        return out
+
+    def applyPreconFP32(self, v):
+        v = _f64(v)
+        out = np.empty(self.ndim)
+        _check(self.lib.iemic_apply_precon_fp32(self.h, _d(v), _d(out)))
         return out
 
     @staticmethod

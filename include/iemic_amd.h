@@ -144,6 +144,9 @@ int iemic_preconditioner_info(iemic_ctx *ctx, iemic_precond_info *info);
 int iemic_build_preconditioner(iemic_ctx *ctx);
 int iemic_apply_precon(iemic_ctx *ctx, const double *v, double *out);
 int iemic_apply_precon_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
+/* the same with the factors computed and applied in FP32 (vectors stay FP64);
+ * the preconditioner of the mixed-precision solve */
+int iemic_apply_precon_fp32(iemic_ctx *ctx, const double *v, double *out);
 
 /* ---- Ocean::solve (Ocean.C:1070-1147): J x = b, x0 = 0 ------------------ */
 void iemic_solver_defaults(iemic_solver_params *p);

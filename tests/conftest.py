@@ -5,6 +5,8 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
+# the product's Python binding (i-emic_amd/python/iemic_amd)
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "i-emic_amd", "python"))
 
 
 def pytest_configure(config):
