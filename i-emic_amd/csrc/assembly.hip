@@ -903,6 +903,60 @@ __global__ void k_export(const int *__restrict__ src, const int *__restrict__ co
     val[p] = (s >= 0) ? J[s] : sign * coef[col[p]];
 }
 
+// Ocean::pressureProjection (Ocean.C:2180-2235): v -= (v.s1) s1 + (v.s2) s2,
+// s1 / s2 the normalised indicators of the pressure unknowns of the wet
+// cells with i + j even / odd (the checkerboard modes, THCM.C:2802-2841).
+// With s = e / sqrt(count), (v.s) s is the mean of v over the class, so the
+// sums and counts of the two classes are reduced (fixed order) and each
+// class mean subtracted.
+__global__ void __launch_bounds__(256) k_pressure_partial(Scalars s, const uint8_t *__restrict__ lm,
+                                                          const double *__restrict__ v,
+                                                          double *__restrict__ part)
+{
+    __shared__ double sh[4][256];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < s.ncell; c += gridDim.x * blockDim.x) {
+        const int i = c % s.n, j = (c / s.n) % s.m, k = c / (s.n * s.m);
+        if (lm[(i + 1) + (s.n + 2) * ((j + 1) + (s.m + 2) * (k + 1))] != OCEAN) continue;
+        const int cls = (i + j) & 1;
+        acc[cls] += v[6 * c + 3];
+        acc[2 + cls] += 1.0;
+    }
+    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void __launch_bounds__(256) k_pressure_final(const double *__restrict__ part, int np,
+                                                        double *__restrict__ mean)
+{
+    __shared__ double sh[4][256];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < np; b += blockDim.x)
+        for (int q = 0; q < 4; q++) acc[q] += part[4 * b + q];
+    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) mean[threadIdx.x] = sh[2 + threadIdx.x][0] > 0.0 ? sh[threadIdx.x][0] / sh[2 + threadIdx.x][0] : 0.0;
+}
+__global__ void __launch_bounds__(256) k_pressure_update(Scalars s, const uint8_t *__restrict__ lm,
+                                                         const double *__restrict__ mean, double *__restrict__ v)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= s.ncell) return;
+    const int i = c % s.n, j = (c / s.n) % s.m, k = c / (s.n * s.m);
+    if (lm[(i + 1) + (s.n + 2) * ((j + 1) + (s.m + 2) * (k + 1))] != OCEAN) return;
+    v[6 * c + 3] -= mean[(i + j) & 1];
+}
+
 // ------------------------------------------------------------------ host side
 static KArgs kargs(iemic_ctx *c)
 {
@@ -1010,6 +1064,19 @@ int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y)
 {
     hipLaunchKernelGGL(k_spmv_t, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc, c->d_J,
                        c->d_coef, (double)c->cfg.int_sign, d_x, d_y);
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+
+int launch_pressure_projection(iemic_ctx *c, double *d_v)
+{
+    // partials: 4 per block in d_red[0 .. 4 nb); class means after them
+    const int nb = std::min(c->nred / 4 - 1, nblk(c->sc.ncell, 256));
+    double *mean = c->d_red + 4 * nb;
+    hipLaunchKernelGGL(k_pressure_partial, dim3(nb), dim3(256), 0, c->stream, c->sc, c->d_landm, d_v, c->d_red);
+    hipLaunchKernelGGL(k_pressure_final, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, mean);
+    hipLaunchKernelGGL(k_pressure_update, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc,
+                       c->d_landm, mean, d_v);
     HIPCHK(hipGetLastError());
     return IEMIC_OK;
 }

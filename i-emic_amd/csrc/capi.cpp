@@ -19,7 +19,8 @@ void precond_free(iemic_ctx *c);
 int precond_check(iemic_ctx *c);
 int precond_info(iemic_ctx *c, iemic_precond_info *info);
 int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
-                 iemic_solve_info *info);
+                 iemic_solve_info *info, bool transpose);
+int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
 void krylov_free(iemic_ctx *c);
 }  // namespace iemic
 
@@ -260,6 +261,28 @@ int iemic_apply_matrix(iemic_ctx *c, const double *v, double *out)
     return IEMIC_OK;
 }
 
+int iemic_pressure_projection_dev(iemic_ctx *c, double *d_v)
+{
+    CHECK_CTX(c);
+    if (!d_v) return set_error(IEMIC_EINVAL, "null pointer");
+    return launch_pressure_projection(c, d_v);
+}
+
+int iemic_pressure_projection(iemic_ctx *c, double *v)
+{
+    CHECK_CTX(c);
+    if (!v) return set_error(IEMIC_EINVAL, "null pointer");
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    size_t bytes = (size_t)c->sc.ndim * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_x, v, bytes, hipMemcpyHostToDevice, c->stream));
+    rc = launch_pressure_projection(c, c->d_x);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(v, c->d_x, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IEMIC_OK;
+}
+
 int iemic_apply_mass(iemic_ctx *c, const double *v, double *out)
 {
     CHECK_CTX(c);
@@ -345,18 +368,18 @@ void iemic_solver_defaults(iemic_solver_params *p)
     p->verbose = 0;
 }
 
-int iemic_solve_dev(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
-                    iemic_solve_info *info)
+static int solve_dev(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
+                     iemic_solve_info *info, bool transpose)
 {
     CHECK_CTX(c);
     if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
     iemic_solver_params def;
     iemic_solver_defaults(&def);
-    return krylov_solve(c, p ? p : &def, d_b, d_x, info);
+    return krylov_solve(c, p ? p : &def, d_b, d_x, info, transpose);
 }
 
-int iemic_solve(iemic_ctx *c, const iemic_solver_params *p, const double *b, double *x,
-                iemic_solve_info *info)
+static int solve_host(iemic_ctx *c, const iemic_solver_params *p, const double *b, double *x,
+                      iemic_solve_info *info, bool transpose)
 {
     CHECK_CTX(c);
     int rc = ensure_staging(c);
@@ -367,13 +390,50 @@ int iemic_solve(iemic_ctx *c, const iemic_solver_params *p, const double *b, dou
     HIPCHK(hipMalloc(&dx, bytes));
     rc = IEMIC_OK;
     if (hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy b");
-    if (!rc) rc = iemic_solve_dev(c, p, db, dx, info);
+    if (!rc) rc = solve_dev(c, p, db, dx, info, transpose);
     if (rc == IEMIC_OK || rc == IEMIC_ENOCONV) {
         if (hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy x");
     }
     (void)hipFree(db);
     (void)hipFree(dx);
     return rc;
+}
+
+int iemic_solve_dev(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
+                    iemic_solve_info *info)
+{
+    return solve_dev(c, p, d_b, d_x, info, false);
+}
+
+int iemic_solve(iemic_ctx *c, const iemic_solver_params *p, const double *b, double *x, iemic_solve_info *info)
+{
+    return solve_host(c, p, b, x, info, false);
+}
+
+int iemic_solve_transpose_dev(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
+                              iemic_solve_info *info)
+{
+    return solve_dev(c, p, d_b, d_x, info, true);
+}
+
+int iemic_solve_transpose(iemic_ctx *c, const iemic_solver_params *p, const double *b, double *x,
+                          iemic_solve_info *info)
+{
+    return solve_host(c, p, b, x, info, true);
+}
+
+int iemic_apply_precon_transpose_dev(iemic_ctx *c, const double *d_v, double *d_out)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    return precond_apply_t(c, 0, d_v, d_out);
+}
+
+int iemic_diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad)
+{
+    CHECK_CTX(c);
+    if (!x || !value) return set_error(IEMIC_EINVAL, "null pointer");
+    return diagnostic(c, which, x, value, grad);
 }
 
 int iemic_dev_alloc(void **ptr, long long bytes)

@@ -75,6 +75,7 @@ int host_init(iemic_ctx *c, const int *landm, const double *taux, const double *
               const double *tatm, const double *emip, const double *spert);
 void host_parameters_changed(iemic_ctx *c);   // forcing + linear tables (setparcs_)
 void build_graph(iemic_ctx *c);
+int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad);
 int set_error(int code, const std::string &msg);
 // assembly.hip
 int upload_tables(iemic_ctx *c);
@@ -83,6 +84,7 @@ int launch_jacobian(iemic_ctx *c, const double *d_x);
 int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_export(iemic_ctx *c, double *d_val);
+int launch_pressure_projection(iemic_ctx *c, double *d_v);
 double device_dot(iemic_ctx *c, const double *d_a, const double *d_b, int n);
 void timed_begin_ext(iemic_ctx *c, int which);
 void timed_end_ext(iemic_ctx *c, int which);

@@ -551,3 +551,75 @@ void build_graph(iemic_ctx *c)
 }
 
 }  // namespace iemic
+
+namespace iemic {
+
+// Steady-state diagnostics for sensitivity studies, with their gradients
+// with respect to the state.
+//  0  area-mean SST: sum over ocean surface cells of cos(y_j) T / sum cos(y_j)
+//     (T is the anomaly unknown; a constant offset does not change gradients)
+//  1  maximum of the meridional overturning streamfunction, the quantity
+//     OceanGrid::psimMax reports (OceanGrid.C:270-345, compute_psim in
+//     thcm_utils.F90:95-118): vs(j,k) = dx sum_i v(i,j,k),
+//     psi(j,k) = -cos(yv_j) vs(j,k) dz dfzT(k) + psi(j,k-1) below 500 m;
+//     the gradient is that of psi at its (first) maximiser.
+int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad)
+{
+    const int n = c->cfg.n, m = c->cfg.m, l = c->cfg.l;
+    const Host &h = c->h;
+    const size_t ndim = (size_t)6 * n * m * l;
+    auto row = [&](int i, int j, int k, int var) {   // 1-based i, j, k
+        return (size_t)6 * ((size_t)(i - 1) + (size_t)n * ((j - 1) + (size_t)m * (k - 1))) + var;
+    };
+    if (grad) std::fill(grad, grad + ndim, 0.0);
+    if (which == 0) {
+        double sw = 0.0, s = 0.0;
+        for (int j = 1; j <= m; j++)
+            for (int i = 1; i <= n; i++)
+                if (lm(c, i, j, l) == OCEAN) sw += std::cos(h.y[j]);
+        if (sw == 0.0) return set_error(IEMIC_EINVAL, "no ocean surface cells");
+        for (int j = 1; j <= m; j++)
+            for (int i = 1; i <= n; i++) {
+                if (lm(c, i, j, l) != OCEAN) continue;
+                const double w = std::cos(h.y[j]) / sw;
+                s += w * x[row(i, j, l, T)];
+                if (grad) grad[row(i, j, l, T)] = w;
+            }
+        *value = s;
+        return IEMIC_OK;
+    }
+    if (which == 1) {
+        double best = 0.0;
+        int bj = -1, bk = -1;
+        std::vector<double> psi((size_t)(m + 1) * (l + 1), 0.0);
+        for (int j = 1; j <= m; j++) {
+            const double cs = std::cos(h.yv[j]);
+            for (int k = 1; k <= l; k++) {
+                if (!(h.z[k] * c->cfg.hdim < -500.0)) continue;
+                double vs = 0.0;
+                for (int i = 1; i <= n; i++) vs += x[row(i, j, k, V)];
+                vs *= h.dx;
+                psi[j + (size_t)(m + 1) * k] = -cs * vs * h.dz * h.dfzT[k] + psi[j + (size_t)(m + 1) * (k - 1)];
+            }
+        }
+        // maximum over (j, k) as the reference scans it, psi(0,0) = 0 included
+        for (int j = 0; j <= m; j++)
+            for (int k = 0; k <= l; k++)
+                if (psi[j + (size_t)(m + 1) * k] > best) {
+                    best = psi[j + (size_t)(m + 1) * k];
+                    bj = j;
+                    bk = k;
+                }
+        *value = best;
+        if (grad && bj >= 1)
+            for (int k = 1; k <= bk; k++) {
+                if (!(h.z[k] * c->cfg.hdim < -500.0)) continue;
+                const double w = -std::cos(h.yv[bj]) * h.dx * h.dz * h.dfzT[k];
+                for (int i = 1; i <= n; i++) grad[row(i, bj, k, V)] = w;
+            }
+        return IEMIC_OK;
+    }
+    return set_error(IEMIC_EINVAL, "unknown diagnostic");
+}
+
+}  // namespace iemic

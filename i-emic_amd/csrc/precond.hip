@@ -817,6 +817,7 @@ static int cilu_apply_f32(iemic_ctx *c, const float *d_in, float *d_out)
 int rowlu_build(iemic_ctx *c, int precision);
 int rowlu_ready(iemic_ctx *c, int precision);
 int rowlu_apply(iemic_ctx *c, int precision, const double *in, double *out);
+int rowlu_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
 void rowlu_free(iemic_ctx *c);
 
 void precond_free(iemic_ctx *c)
@@ -851,6 +852,24 @@ int precond_apply(iemic_ctx *c, int precision, const double *in, double *out, fl
     hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(n, 256)), dim3(256), 0, c->stream, out, tmp32 + n, n);
     HIPCHK(hipGetLastError());
     return IEMIC_OK;
+}
+
+// z = M^{-T} v (adjoint solves); the row LU only
+int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out)
+{
+    if (c->prec_kind != IEMIC_PREC_ROW_LU)
+        return set_error(IEMIC_EUNSUPPORTED, "transposed preconditioner: row LU only");
+    return rowlu_apply_t(c, precision, in, out);
+}
+
+int rowlu_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count);
+
+// null vectors of J per pressure basin (row LU only; none for the column ILU)
+int precond_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count)
+{
+    *count = 0;
+    if (c->prec_kind != IEMIC_PREC_ROW_LU) return IEMIC_OK;
+    return rowlu_null_space(c, precision, left, vecs, nrm2, count);
 }
 
 int precond_check(iemic_ctx *c) { return c->prec_kind == IEMIC_PREC_ROW_LU ? IEMIC_OK : cilu_check(c); }

@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "ctx.hpp"
@@ -43,6 +44,8 @@
 namespace iemic {
 
 std::vector<int> column_tops(const iemic_ctx *c);
+template <typename T>
+int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info);
 std::vector<uint8_t> pressure_pins(const iemic_ctx *c, const std::vector<int> &k0, const std::vector<int> &rank);
 
 template <typename T>
@@ -61,6 +64,8 @@ struct RowLU {
     int *d_perm = nullptr;               // per ocean unknown: model index 6 cell + var
     int *d_lptr = nullptr;               // A_{j,j-1} by rows of row j (CSR over ocean unknowns)
     int2 *d_lent = nullptr;              //   {local column in row j-1, slot-value index}
+    int *d_cptr = nullptr;               // A_{j+1,j} by columns of row j (transposed sweep)
+    int2 *d_cent = nullptr;              //   {local row in row j+1, index of the entry in lent/lval}
     int *d_uptr = nullptr;               // A_{j-1,j} by columns in row j
     int2 *d_uent = nullptr;              //   {local row in row j-1, slot-value index}
     uint8_t *d_pin = nullptr;            // per cell: continuity row replaced by p = 0
@@ -73,9 +78,23 @@ struct RowLU {
     void *d_v = nullptr, *d_t = nullptr;  // row-layout work vectors (nocean, nmax) in the widest type
     void *d_w = nullptr;                  // nmax^2 scratch: right-hand sides of the inversion
     rocblas_handle blas = nullptr;
+    // null vectors of J, one per pressure basin b (pinned row r_b):
+    // left m_b = M^{-T} e_{r_b}, right n_b = M^{-1} e_{r_b}
+    std::vector<int> pin_rows;
+    int *d_pin_rows = nullptr;
+    double *d_null[2] = {nullptr, nullptr};   // [0] left, [1] right: npins x ndim
+    std::vector<double> null_nrm2[2];
+    unsigned null_version[2] = {0, 0};
+    int null_prec[2] = {-1, -1};
+    double *d_unit = nullptr;
+    bool dense_lu = false;                // own dense_getrf instead of rocSOLVER getrf (comparison runs)
 };
 
 static inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+// at most this many pressure basins get explicit null vectors (two vectors
+// of ndim doubles each)
+constexpr int ROWLU_NULL_MAX = 64;
 
 struct RowGeom {
     const int *loc;
@@ -303,6 +322,19 @@ __global__ void k_row_lmv(int nrow, const int *__restrict__ lptr, const int2 *__
     t[e] = acc;
 }
 
+// t = y_j - A_{j+1,j}^T x_{j+1}  (entries of A_{j+1,j} listed by column of row j)
+template <typename T>
+__global__ void k_row_ltmv(int nrow, const int *__restrict__ cptr, const int2 *__restrict__ cent,
+                           const double *__restrict__ lval, const T *__restrict__ yj, const T *__restrict__ xnext,
+                           T *__restrict__ t)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrow) return;
+    T acc = yj[e];
+    for (int q = cptr[e]; q < cptr[e + 1]; q++) acc -= (T)lval[cent[q].y] * xnext[cent[q].x];
+    t[e] = acc;
+}
+
 // ------------------------------------------------------------------ host
 #define BLASCHK(expr)                                                                  \
     do {                                                                               \
@@ -316,9 +348,10 @@ void rowlu_free(iemic_ctx *c)
 {
     RowLU *p = c->rowlu;
     if (!p) return;
-    void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_uptr, p->d_uent, p->d_pin,
+    void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_cptr, p->d_cent, p->d_uptr, p->d_uent, p->d_pin,
                     p->d_rs, p->d_cs, p->d_lval, p->d_uval,
-                    p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t, p->d_w};
+                    p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t, p->d_w,
+                    p->d_null[0], p->d_null[1], p->d_unit, p->d_pin_rows};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->blas) rocblas_destroy_handle(p->blas);
@@ -340,7 +373,9 @@ static int rowlu_ensure(iemic_ctx *c)
     std::vector<int> rank((size_t)n * m);
     for (int q = 0; q < n * m; q++) rank[q] = q;   // rows eliminated in j order
     const std::vector<uint8_t> pin = pressure_pins(c, k0, rank);
-    for (uint8_t v : pin) p->npins += v;
+    for (size_t q = 0; q < C; q++)
+        if (pin[q]) p->pin_rows.push_back(6 * (int)q + P);
+    p->npins = (int)p->pin_rows.size();
     // local numbering within rows and the ocean-unknown permutation
     std::vector<int> loc(C, -1), perm;
     p->nj.assign(m, 0);
@@ -410,6 +445,21 @@ static int rowlu_ensure(iemic_ctx *c)
     }
     lptr[p->nocean] = (int)lent.size();
     uptr[p->nocean] = (int)uent.size();
+    // A_{j+1,j} by columns of row j, pointing at the lent entries
+    std::vector<std::vector<int2>> lcol(p->nocean);
+    for (int j = 1; j < m; j++)
+        for (int e = 0; e < p->nj[j]; e++) {
+            const int u = p->ubase[j] + e;
+            for (int q = lptr[u]; q < lptr[u + 1]; q++) lcol[p->ubase[j - 1] + lent[q].x].push_back(make_int2(e, q));
+        }
+    std::vector<int> cptr(p->nocean + 1, 0);
+    std::vector<int2> cent;
+    for (int u = 0; u < p->nocean; u++) {
+        cptr[u] = (int)cent.size();
+        cent.insert(cent.end(), lcol[u].begin(), lcol[u].end());
+    }
+    cptr[p->nocean] = (int)cent.size();
+    if (cent.empty()) cent.push_back(make_int2(0, 0));
     if (lent.empty()) lent.push_back(make_int2(0, 0));
     if (uent.empty()) uent.push_back(make_int2(0, 0));
     auto up = [&](auto **dst, const auto &v) -> int {
@@ -420,7 +470,8 @@ static int rowlu_ensure(iemic_ctx *c)
     int rc;
     if ((rc = up(&p->d_loc, loc)) || (rc = up(&p->d_perm, perm)) || (rc = up(&p->d_lptr, lptr)) ||
         (rc = up(&p->d_lent, lent)) || (rc = up(&p->d_uptr, uptr)) || (rc = up(&p->d_uent, uent)) ||
-        (rc = up(&p->d_pin, pin)))
+        (rc = up(&p->d_cptr, cptr)) || (rc = up(&p->d_cent, cent)) ||
+        (rc = up(&p->d_pin, pin)) || (!p->pin_rows.empty() && (rc = up(&p->d_pin_rows, p->pin_rows))))
         return rc;
     p->nl = (int)lent.size();
     p->nu = (int)uent.size();
@@ -432,6 +483,7 @@ static int rowlu_ensure(iemic_ctx *c)
     HIPCHK(hipMalloc(&p->d_info, (size_t)m * sizeof(int)));
     HIPCHK(hipMalloc(&p->d_v, (size_t)std::max(p->nocean, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&p->d_t, (size_t)std::max(p->nmax, 1) * sizeof(double)));
+    if (const char *v = std::getenv("IEMIC_DENSE_LU")) p->dense_lu = std::atoi(v) != 0;
     BLASCHK(rocblas_create_handle(&p->blas));
     BLASCHK(rocblas_set_stream(p->blas, c->stream));
     // size the handle's workspace once for the largest row, so that the
@@ -482,6 +534,17 @@ static rocblas_status gemv(rocblas_handle h, int rows, int cols, float alpha, co
     return rocblas_sgemv(h, rocblas_operation_none, rows, cols, &a, A, rows, x, 1, &b, y, 1);
 }
 
+static rocblas_status gemv_t(rocblas_handle h, int rows, int cols, double alpha, const double *A, const double *x,
+                             double beta, double *y)
+{
+    return rocblas_dgemv(h, rocblas_operation_transpose, rows, cols, &alpha, A, rows, x, 1, &beta, y, 1);
+}
+static rocblas_status gemv_t(rocblas_handle h, int rows, int cols, float alpha, const float *A, const float *x,
+                             float beta, float *y)
+{
+    return rocblas_sgemv(h, rocblas_operation_transpose, rows, cols, &alpha, A, rows, x, 1, &beta, y, 1);
+}
+
 template <typename T>
 static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
 {
@@ -518,7 +581,12 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
                                p->d_lent, p->d_lval, X, D);
         }
         HIPCHK(hipGetLastError());
-        BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
+        if (p->dense_lu) {
+            int rc = dense_getrf<T>(p->blas, c->stream, nr, D, p->d_ipiv, p->d_info + j);
+            if (rc) return rc;
+        } else {
+            BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
+        }
         // Minv_j: solve D W = I, then W replaces the factors
         T *W = reinterpret_cast<T *>(p->d_w);
         hipLaunchKernelGGL(k_identity<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 8192)), dim3(256), 0, c->stream, W,
@@ -572,6 +640,42 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
     return IEMIC_OK;
 }
 
+// M^{-T}: with M = L U, L block lower (D_j on the diagonal, A_{j,j-1} below)
+// and U unit block upper (X_{j+1} above), M^{-T} = L^{-T} U^{-T}:
+//   forward  y_0 = v_0,  y_{j+1} = v_{j+1} - X_{j+1}^T y_j
+//   backward x_j = D_j^{-T} (y_j - A_{j+1,j}^T x_{j+1})
+// and the scales swap roles: M^{-T} = R (R M C)^{-T} C.
+template <typename T>
+static int rowlu_sweep_t(iemic_ctx *c, const RowFactors<T> &F, const double *in, double *out)
+{
+    RowLU *p = c->rowlu;
+    T *v = reinterpret_cast<T *>(p->d_v), *t = reinterpret_cast<T *>(p->d_t);
+    hipLaunchKernelGGL(k_row_land, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc.ncell, p->d_loc, in, out);
+    if (p->nocean == 0) return IEMIC_OK;
+    hipLaunchKernelGGL(k_row_gather<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
+                       p->d_cs, in, v);
+    for (int j = 0; j + 1 < p->m; j++) {
+        const int nr = p->nj[j], nn = p->nj[j + 1];
+        if (nr == 0 || nn == 0) continue;
+        BLASCHK(gemv_t(p->blas, nr, nn, (T)-1, F.blk + p->xoff[j + 1], v + p->ubase[j], (T)1, v + p->ubase[j + 1]));
+    }
+    for (int j = p->m - 1; j >= 0; j--) {
+        const int nr = p->nj[j];
+        if (nr == 0) continue;
+        T *vj = v + p->ubase[j];
+        if (j + 1 < p->m && p->nj[j + 1] > 0)
+            hipLaunchKernelGGL(k_row_ltmv<T>, dim3(nblk(nr, 256)), dim3(256), 0, c->stream, nr,
+                               p->d_cptr + p->ubase[j], p->d_cent, p->d_lval, vj, v + p->ubase[j + 1], t);
+        else
+            HIPCHK(hipMemcpyAsync(t, vj, (size_t)nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        BLASCHK(gemv_t(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
+    }
+    hipLaunchKernelGGL(k_row_scatter<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
+                       p->d_rs, v, out);
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+
 static bool row_current(const iemic_ctx *c, bool built, unsigned version)
 {
     return built && version == c->jac_version;
@@ -608,6 +712,143 @@ int rowlu_apply(iemic_ctx *c, int precision, const double *in, double *out)
     rc = precision ? rowlu_sweep<float>(c, p->f32, in, out) : rowlu_sweep<double>(c, p->f64, in, out);
     timed_end_ext(c, precision ? 6 : 3);
     return rc;
+}
+
+int rowlu_apply_t(iemic_ctx *c, int precision, const double *in, double *out)
+{
+    int rc = rowlu_ready(c, precision);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    return precision ? rowlu_sweep_t<float>(c, p->f32, in, out) : rowlu_sweep_t<double>(c, p->f64, in, out);
+}
+
+__global__ void k_unit(double *__restrict__ v, int n, int r)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) v[q] = q == r ? 1.0 : 0.0;
+}
+__global__ void k_sub_from(double *__restrict__ y, const double *__restrict__ x, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q] -= x[q];
+}
+__global__ void k_add_to(double *__restrict__ y, const double *__restrict__ x, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q] += x[q];
+}
+__global__ void k_rsub(double *__restrict__ y, const double *__restrict__ t, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q] = t[q] - y[q];
+}
+// the rows M replaces (pins, then the integral row if any): y[r] = mode 0:
+// v[r] (pins) / d v[r] (integral row); mode 1: 0; mode 2: y[r] + (same as 0)
+__global__ void k_replaced_rows(double *__restrict__ y, const double *__restrict__ v, const int *__restrict__ rows,
+                                int nrows, int ric, double d, int mode)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > nrows) return;
+    int r;
+    double w;
+    if (q < nrows) {
+        r = rows[q];
+        w = v[r];
+    } else {
+        if (ric < 0) return;
+        r = ric;
+        w = d * v[r];
+    }
+    y[r] = mode == 1 ? 0.0 : (mode == 2 ? y[r] + w : w);
+}
+
+// y = M v (or M^T v) through products with J: M = J + sum_c e_{r_c} (e_{r_c}
+// - J[r_c,:]) + e_ric (d e_ric - J[ric,:]), so M v is J v with the replaced
+// rows set to v[r_c] and d v[ric], and M^T v = J^T v~ plus those diagonal
+// terms, v~ = v with the replaced entries zeroed.  tmp: ndim scratch.
+static int apply_pinned(iemic_ctx *c, RowLU *p, bool transpose, const double *v, double *y, double *tmp)
+{
+    const int nd = c->sc.ndim, ric = c->sc.rowintcon, nr = p->npins;
+    const double d = ric >= 0 && c->h.coef[ric] != 0.0 ? c->cfg.int_sign * c->h.coef[ric] : 1.0;
+    int rc;
+    if (!transpose) {
+        if ((rc = launch_spmv(c, v, y))) return rc;
+        hipLaunchKernelGGL(k_replaced_rows, dim3(nblk(nr + 1, 64)), dim3(64), 0, c->stream, y, v, p->d_pin_rows, nr,
+                           ric, d, 0);
+    } else {
+        HIPCHK(hipMemcpyAsync(tmp, v, (size_t)nd * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_replaced_rows, dim3(nblk(nr + 1, 64)), dim3(64), 0, c->stream, tmp, v, p->d_pin_rows, nr,
+                           ric, d, 1);
+        if ((rc = launch_spmv_t(c, tmp, y))) return rc;
+        hipLaunchKernelGGL(k_replaced_rows, dim3(nblk(nr + 1, 64)), dim3(64), 0, c->stream, y, v, p->d_pin_rows, nr,
+                           ric, d, 2);
+    }
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+
+template <typename T>
+static int pinned_solve(iemic_ctx *c, RowLU *p, const RowFactors<T> &F, bool transpose, const double *in, double *out)
+{
+    return transpose ? rowlu_sweep_t<T>(c, F, in, out) : rowlu_sweep<T>(c, F, in, out);
+}
+
+// Null vectors of J, one per pressure basin b with pinned row r_b.  The
+// factored matrix M equals J except in the replaced rows (see apply_pinned).
+// A right null vector n of J with n[r_c] = delta_bc satisfies M n = e_{r_b};
+// a left null vector m of J (m^T J = 0, supported on the continuity rows of
+// basin b, m[r_c] = delta_bc, m[ric] = 0) satisfies M^T m = e_{r_b} - J^T
+// e_{r_b}.  Each is solved with the factors of the given precision, refined
+// twice against FP64 products with M, and kept only if ||J n|| (||J^T m||)
+// is below 1e-10 ||J e_r|| ||n||: a basin whose vector fails that test gets
+// none (its norm is stored as 0).  Computed after each factorisation, on
+// demand.
+int rowlu_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count)
+{
+    *count = 0;
+    int rc = rowlu_ready(c, precision);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    const int side = left ? 0 : 1, nb = p->npins;
+    const size_t nd = (size_t)c->sc.ndim;
+    if (nb == 0 || nb > ROWLU_NULL_MAX) return IEMIC_OK;
+    const unsigned version = precision ? p->f32.version : p->f64.version;
+    if (p->null_prec[side] != precision || p->null_version[side] != version) {
+        if (!p->d_null[side]) HIPCHK(hipMalloc(&p->d_null[side], (size_t)nb * nd * sizeof(double)));
+        if (!p->d_unit) HIPCHK(hipMalloc(&p->d_unit, 3 * nd * sizeof(double)));
+        double *t = p->d_unit, *w1 = p->d_unit + nd, *w2 = p->d_unit + 2 * nd;
+        const dim3 g(nblk(nd, 256)), blk(256);
+        p->null_nrm2[side].assign(nb, 0.0);
+        for (int b = 0; b < nb; b++) {
+            double *v = p->d_null[side] + (size_t)b * nd;
+            // t = e_r (right) or e_r - J^T e_r (left); w2 = J e_r / J^T e_r
+            hipLaunchKernelGGL(k_unit, g, blk, 0, c->stream, t, (int)nd, p->pin_rows[b]);
+            if ((rc = left ? launch_spmv_t(c, t, w2) : launch_spmv(c, t, w2))) return rc;
+            const double colnorm = std::sqrt(device_dot(c, w2, w2, (int)nd));
+            if (left) hipLaunchKernelGGL(k_sub_from, g, blk, 0, c->stream, t, w2, (int)nd);
+            rc = precision ? pinned_solve<float>(c, p, p->f32, left, t, v) : pinned_solve<double>(c, p, p->f64, left, t, v);
+            if (rc) return rc;
+            for (int it = 0; it < 2; it++) {
+                if ((rc = apply_pinned(c, p, left, v, w1, w2))) return rc;
+                hipLaunchKernelGGL(k_rsub, g, blk, 0, c->stream, w1, t, (int)nd);
+                rc = precision ? pinned_solve<float>(c, p, p->f32, left, w1, w2)
+                               : pinned_solve<double>(c, p, p->f64, left, w1, w2);
+                if (rc) return rc;
+                hipLaunchKernelGGL(k_add_to, g, blk, 0, c->stream, v, w2, (int)nd);
+            }
+            if ((rc = left ? launch_spmv_t(c, v, w1) : launch_spmv(c, v, w1))) return rc;
+            HIPCHK(hipGetLastError());
+            const double vv = device_dot(c, v, v, (int)nd), res = std::sqrt(device_dot(c, w1, w1, (int)nd));
+            if (std::isnan(vv) || std::isnan(res)) return set_error(IEMIC_EHIP, "null vectors: reduction failed");
+            p->null_nrm2[side][b] = res <= 1e-10 * colnorm * std::sqrt(vv) ? vv : 0.0;
+        }
+        p->null_prec[side] = precision;
+        p->null_version[side] = version;
+    }
+    *vecs = p->d_null[side];
+    *nrm2 = p->null_nrm2[side].data();
+    *count = nb;
+    return IEMIC_OK;
 }
 
 int rowlu_info(iemic_ctx *c, iemic_precond_info *info)

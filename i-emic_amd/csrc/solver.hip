@@ -27,6 +27,8 @@ namespace iemic {
 int precond_ready(iemic_ctx *c, int precision);
 int precond_apply(iemic_ctx *c, int precision, const double *in, double *out, float *tmp32);
 int precond_check(iemic_ctx *c);
+int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
+int precond_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count);
 
 template <typename T>
 struct Basis {
@@ -40,9 +42,11 @@ struct Krylov {
     double *d_part = nullptr;  // partial sums (blocks x (restart+2))
     int nblocks = 0;
     double *d_r = nullptr;     // FP64 residual
+    double *d_bp = nullptr;    // right-hand side with its inconsistent part removed
     float *tmp32 = nullptr;    // 2 ndim: FP32 preconditioner input/output
     int hcap = 0;
     int precision = 0;         // preconditioner precision of the current solve
+    bool transpose = false;    // solving J^T x = b (adjoint)
 };
 
 // ------------------------------------------------------------------ kernels
@@ -134,7 +138,7 @@ void krylov_free(iemic_ctx *c)
 {
     Krylov *k = c->kry;
     if (!k) return;
-    void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->d_h, k->d_part, k->d_r, k->tmp32};
+    void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->d_h, k->d_part, k->d_r, k->d_bp, k->tmp32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete k;
@@ -172,9 +176,13 @@ static int ensure_krylov(iemic_ctx *c, int m)
 // matrix and preconditioner of the iteration
 template <typename T> struct Ops;
 template <> struct Ops<double> {
-    static int matvec(iemic_ctx *c, const double *x, double *y) { return launch_spmv(c, x, y); }
+    static int matvec(iemic_ctx *c, const double *x, double *y)
+    {
+        return c->kry->transpose ? launch_spmv_t(c, x, y) : launch_spmv(c, x, y);
+    }
     static int prec(iemic_ctx *c, const double *x, double *y)
     {
+        if (c->kry->transpose) return precond_apply_t(c, c->kry->precision, x, y);
         return precond_apply(c, c->kry->precision, x, y, c->kry->tmp32);
     }
 };
@@ -305,7 +313,7 @@ static int fgmres(iemic_ctx *c, const T *b, T *x, int m, double tol, int maxit, 
 // FP64 true residual r = b - J x and its norm ratio
 static int true_residual(iemic_ctx *c, const double *b, const double *x, double *r, double bnorm, double *rel)
 {
-    int rc = launch_spmv(c, x, r);
+    int rc = Ops<double>::matvec(c, x, r);
     if (rc) return rc;
     hipLaunchKernelGGL(k_axpby, dim3(nblk(c->sc.ndim, 256)), dim3(256), 0, c->stream, r, b, 1.0, -1.0, c->sc.ndim);
     double rn;
@@ -314,8 +322,42 @@ static int true_residual(iemic_ctx *c, const double *b, const double *x, double 
     return IEMIC_OK;
 }
 
+// b' = b minus its components along the null vectors of J^T (J^T x = b:
+// along those of J), one per pressure basin.  J x = b is solvable only for
+// b orthogonal to them; a residual F(x) carries rounding of order eps times
+// the divergence terms in that direction, which near a root of F is no
+// longer small relative to ||F||.  FGMRES on the unprojected b cannot pass
+// that level and its iterates gather large null-space components.  The
+// vectors of different basins have disjoint supports, so the projections
+// are applied one after the other.
+static int consistent_rhs(iemic_ctx *c, int precision, bool transpose, const double *b, const double **out)
+{
+    *out = b;
+    const double *vecs, *nrm2;
+    int cnt;
+    int rc = precond_null_space(c, precision, !transpose, &vecs, &nrm2, &cnt);
+    if (rc || cnt == 0) return rc;
+    Krylov *k = c->kry;
+    const int n = c->sc.ndim;
+    if (!k->d_bp) HIPCHK(hipMalloc(&k->d_bp, (size_t)n * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(k->d_bp, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    for (int q = 0; q < cnt; q++) {
+        if (!(nrm2[q] > 0.0)) continue;
+        const double *v = vecs + (size_t)q * n;
+        const double a = device_dot(c, v, k->d_bp, n) / nrm2[q];
+        if (std::isnan(a)) return set_error(IEMIC_EHIP, "null-space projection: reduction failed");
+        hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, k->d_bp, v, -a, 1.0, n);
+    }
+    HIPCHK(hipGetLastError());
+    *out = k->d_bp;
+    return IEMIC_OK;
+}
+
+// J x = b, or J^T x = b when transpose is set (the adjoint solve: transposed
+// products and the transposed preconditioner, same FGMRES).  FGMRES runs on
+// the consistent part of b; the reported residual is that of b itself.
 int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
-                 iemic_solve_info *info)
+                 iemic_solve_info *info, bool transpose)
 {
     using clk = std::chrono::steady_clock;
     if (p->restart < 1 || p->maxit < 1 || p->tol <= 0.0 || (p->precision != 0 && p->precision != 1))
@@ -324,6 +366,7 @@ int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, 
     if (rc) return rc;
     Krylov *k = c->kry;
     k->precision = p->precision;
+    k->transpose = transpose;
     iemic_solve_info inf{};
     auto t0 = clk::now();
     if ((rc = precond_ready(c, p->precision))) return rc;
@@ -332,9 +375,11 @@ int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, 
     inf.t_precon_setup = std::chrono::duration<double>(t1 - t0).count();
     double bnorm;
     if ((rc = norm2(c, d_b, c->sc.ndim, bnorm))) return rc;
+    const double *d_bc;
+    if ((rc = consistent_rhs(c, p->precision, transpose, d_b, &d_bc))) return rc;
     int its, cycles;
     double est;
-    if ((rc = fgmres<double>(c, d_b, d_x, p->restart, p->tol, p->maxit, p->verbose, &its, &cycles, &est))) return rc;
+    if ((rc = fgmres<double>(c, d_bc, d_x, p->restart, p->tol, p->maxit, p->verbose, &its, &cycles, &est))) return rc;
     inf.iterations = its;
     inf.restarts = cycles;
     if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &inf.rel_residual))) return rc;

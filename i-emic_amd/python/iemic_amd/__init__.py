@@ -57,6 +57,7 @@ class SolveInfo(C.Structure):
 
 
 PREC_ROW_LU, PREC_COLUMN_ILU = 0, 1
+DIAG_MEAN_SST, DIAG_MAX_PSIM = 0, 1
 
 
 class PrecondInfo(C.Structure):
@@ -99,6 +100,8 @@ def load():
         "iemic_apply_matrix_dev": ([vp, vp, vp], C.c_int),
         "iemic_apply_matrix_transpose_dev": ([vp, vp, vp], C.c_int),
         "iemic_apply_mass": ([vp, dp, dp], C.c_int),
+        "iemic_pressure_projection": ([vp, dp], C.c_int),
+        "iemic_pressure_projection_dev": ([vp, vp], C.c_int),
         "iemic_set_preconditioner": ([vp, C.c_int], C.c_int),
         "iemic_preconditioner_info": ([vp, _P(PrecondInfo)], C.c_int),
         "iemic_build_preconditioner": ([vp], C.c_int),
@@ -108,6 +111,10 @@ def load():
         "iemic_solver_defaults": ([_P(SolverParams)], None),
         "iemic_solve": ([vp, _P(SolverParams), dp, dp, _P(SolveInfo)], C.c_int),
         "iemic_solve_dev": ([vp, _P(SolverParams), vp, vp, _P(SolveInfo)], C.c_int),
+        "iemic_solve_transpose": ([vp, _P(SolverParams), dp, dp, _P(SolveInfo)], C.c_int),
+        "iemic_solve_transpose_dev": ([vp, _P(SolverParams), vp, vp, _P(SolveInfo)], C.c_int),
+        "iemic_apply_precon_transpose_dev": ([vp, vp, vp], C.c_int),
+        "iemic_diagnostic": ([vp, C.c_int, dp, dp, dp], C.c_int),
         "iemic_dev_alloc": ([_P(vp), C.c_longlong], C.c_int),
         "iemic_dev_free": ([vp], C.c_int),
         "iemic_memcpy_h2d": ([vp, vp, C.c_longlong], C.c_int),
@@ -290,6 +297,13 @@ class Ocean:
         _check(self.lib.iemic_apply_mass(self.h, _d(v), _d(out)))
         return out
 
+    def pressureProjection(self, v):
+        """v - (v.s1) s1 - (v.s2) s2 over the checkerboard pressure modes
+        (Ocean::pressureProjection); returns a new array."""
+        v = np.array(v, dtype=np.float64, copy=True)
+        _check(self.lib.iemic_pressure_projection(self.h, _d(v)))
+        return v
+
     def setPreconditioner(self, kind):
         _check(self.lib.iemic_set_preconditioner(self.h, int(kind)))
 
@@ -352,6 +366,32 @@ class Ocean:
 
     def synchronize(self):
         _check(self.lib.iemic_synchronize(self.h))
+
+    def solveTranspose(self, b, params=None, raise_on_noconv=True):
+        """Adjoint solve J^T x = b from x0 = 0; returns (x, SolveInfo)."""
+        b = _f64(b)
+        x = np.zeros(self.ndim)
+        info = SolveInfo()
+        p = params if params is not None else self.solver_params()
+        rc = self.lib.iemic_solve_transpose(self.h, C.byref(p), _d(b), _d(x), C.byref(info))
+        if rc != 0 and (raise_on_noconv or rc != -6):
+            _check(rc)
+        return x, info
+
+    def applyPreconTranspose(self, v):
+        dv = DeviceArray(self.ndim).upload(v)
+        do = DeviceArray(self.ndim)
+        _check(self.lib.iemic_apply_precon_transpose_dev(self.h, dv.ptr, do.ptr))
+        _check(self.lib.iemic_synchronize(self.h))
+        return do.download()
+
+    def diagnostic(self, which, x):
+        """(g(x), dg/dx) for DIAG_MEAN_SST or DIAG_MAX_PSIM."""
+        x = _f64(x)
+        v = C.c_double()
+        g = np.empty(self.ndim)
+        _check(self.lib.iemic_diagnostic(self.h, int(which), _d(x), C.byref(v), _d(g)))
+        return v.value, g
 
     def kernel_ms(self, which):
         v = C.c_double()

@@ -137,6 +137,14 @@ int iemic_apply_matrix_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
 int iemic_apply_matrix_transpose_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
 int iemic_apply_mass(iemic_ctx *ctx, const double *v, double *out);
 
+/* ---- Ocean::pressureProjection (Ocean.C:2180-2235) ---------------------- */
+/* v -= (v.s1) s1 + (v.s2) s2 in place, s1 / s2 the normalised checkerboard
+ * pressure modes (wet cells with i + j even / odd, THCM.C:2802-2841), the
+ * null vectors of J on basins whose wet cells connect through the B-grid
+ * diagonals. */
+int iemic_pressure_projection(iemic_ctx *ctx, double *v);
+int iemic_pressure_projection_dev(iemic_ctx *ctx, double *d_v);
+
 /* ---- Ocean::buildPreconditioner / applyPrecon (Ocean.C:1377-1416) ------ */
 /* choose the preconditioner (IEMIC_PREC_*); drops existing factors */
 int iemic_set_preconditioner(iemic_ctx *ctx, int kind);
@@ -154,6 +162,24 @@ int iemic_solve(iemic_ctx *ctx, const iemic_solver_params *p, const double *b, d
                 iemic_solve_info *info);
 int iemic_solve_dev(iemic_ctx *ctx, const iemic_solver_params *p, const double *d_b,
                     double *d_x, iemic_solve_info *info);
+/* adjoint solve J^T x = b (no counterpart in the reference, whose
+ * preconditioner rejects transposition, TRIOS_BlockPreconditioner.H:122):
+ * FGMRES with J^T products and the transposed row-LU preconditioner; for
+ * sensitivities dg/dp = -x^T dF/dp with b = dg/dx */
+int iemic_solve_transpose(iemic_ctx *ctx, const iemic_solver_params *p, const double *b,
+                          double *x, iemic_solve_info *info);
+int iemic_solve_transpose_dev(iemic_ctx *ctx, const iemic_solver_params *p, const double *d_b,
+                              double *d_x, iemic_solve_info *info);
+int iemic_apply_precon_transpose_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
+
+/* ---- steady-state diagnostics and their state gradients (host arrays) --- */
+/* which = IEMIC_DIAG_MEAN_SST: area-mean surface T (anomaly);
+ * which = IEMIC_DIAG_MAX_PSIM: maximum meridional overturning streamfunction,
+ * as OceanGrid::psimMax (OceanGrid.C:270-345, compute_psim thcm_utils.F90:95-118),
+ * non-dimensional.  grad (length ndim, may be NULL) = dg/dx, the right-hand
+ * side of the adjoint solve. */
+enum { IEMIC_DIAG_MEAN_SST = 0, IEMIC_DIAG_MAX_PSIM = 1 };
+int iemic_diagnostic(const iemic_ctx *ctx, int which, const double *x, double *value, double *grad);
 
 /* ---- device memory helpers for callers without their own allocator ----- */
 int iemic_dev_alloc(void **ptr, long long bytes);

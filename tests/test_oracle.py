@@ -117,3 +117,27 @@ def test_jacobian_columns_against_central_differences():
         diff = np.abs(jc - fd[:, q])[~land]
         scale = max(np.abs(jc).max(), 1.0)
         assert diff.max() < 2e-5 * scale, (c, diff.max(), scale)
+
+
+@pytest.mark.parametrize("name", ["natl8", "box16"])
+def test_checkerboard_modes_span_the_null_space(name):
+    """The pressure modes that Ocean::pressureProjection removes are null
+    vectors of J (J s = 0 to rounding); on natl8 (dense SVD) they span the
+    whole null space, and the projection is idempotent."""
+    import scipy.sparse as sp
+    from thcm_cases import checkerboard_modes, pressure_projection
+    case, _ = load_golden(name)
+    orc = OracleModel(case)
+    rp, ci, val, _, _ = orc.jacobian(case.x)
+    J = sp.csr_matrix((val, ci, rp), shape=(case.ndim, case.ndim))
+    s = checkerboard_modes(case.cfg, orc.landm())
+    assert np.linalg.norm(J @ s.T, axis=0).max() <= 1e-13 * abs(J).max()
+    v = np.random.default_rng(2).normal(size=case.ndim)
+    pv = pressure_projection(case.cfg, orc.landm(), v)
+    assert np.abs(pressure_projection(case.cfg, orc.landm(), pv) - pv).max() <= 1e-14
+    assert np.abs(J @ pv - J @ v).max() <= 1e-12 * (abs(J) @ np.abs(v)).max()
+    if case.ndim <= 2000:
+        _, S, Vt = np.linalg.svd(J.toarray())
+        null = Vt[S <= 1e-14 * S[0]]
+        assert null.shape[0] == 2
+        assert np.linalg.norm(null - (null @ s.T) @ s) <= 1e-8

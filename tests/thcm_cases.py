@@ -146,6 +146,9 @@ class OracleModel:
     def par(self, name):
         return self.lib.tho_get_par(self.h, PAR[name])
 
+    def set_par(self, name, value):
+        assert self.lib.tho_set_par(self.h, PAR[name], float(value)) == 0
+
     def rowintcon(self):
         return self.lib.tho_rowintcon(self.h)
 
@@ -380,6 +383,25 @@ def rand_state(cfg, seed, amp=1.0):
     x[..., 4] = np.linspace(-11, 0, l)[:, None, None] + rng.normal(0, 1.0 * amp, (l, m, n))
     x[..., 5] = rng.normal(0, 0.3 * amp, (l, m, n))
     return x.reshape(-1)
+
+
+def checkerboard_modes(cfg, landm):
+    """The two normalised checkerboard pressure modes s1 (i + j even) and s2
+    (odd) over the wet cells (Ocean.C:2193-2223, THCM.C:2802-2841); landm is
+    the bordered (l+2)(m+2)(n+2) mask."""
+    n, m, l = cfg.n, cfg.m, cfg.l
+    wet = np.asarray(landm).reshape(l + 2, m + 2, n + 2)[1:-1, 1:-1, 1:-1] == 0
+    k, j, i = np.nonzero(wet)
+    rows = 6 * ((k * m + j) * n + i) + 3
+    s = np.zeros((2, 6 * n * m * l))
+    s[(i + j) % 2, rows] = 1.0
+    return s / np.sqrt(s.sum(axis=1))[:, None]
+
+
+def pressure_projection(cfg, landm, v):
+    """Ocean::pressureProjection (Ocean.C:2180-2235): v - (v.s1) s1 - (v.s2) s2."""
+    s = checkerboard_modes(cfg, landm)
+    return v - s.T @ (s @ v)
 
 
 def golden_names():
