@@ -793,16 +793,31 @@ static int pinned_solve(iemic_ctx *c, RowLU *p, const RowFactors<T> &F, bool tra
     return transpose ? rowlu_sweep_t<T>(c, F, in, out) : rowlu_sweep<T>(c, F, in, out);
 }
 
+// ||J v|| (J^T v when left) <= 1e-10 ||J e_r|| ||v||: v is a null vector
+static int null_check(iemic_ctx *c, bool left, const double *v, double colnorm, double *w, bool *ok, double *vv)
+{
+    int rc = left ? launch_spmv_t(c, v, w) : launch_spmv(c, v, w);
+    if (rc) return rc;
+    const int nd = c->sc.ndim;
+    *vv = device_dot(c, v, v, nd);
+    const double res = std::sqrt(device_dot(c, w, w, nd));
+    HIPCHK(hipGetLastError());
+    *ok = res <= 1e-10 * colnorm * std::sqrt(*vv);   // false for NaN (factors that broke down)
+    return IEMIC_OK;
+}
+
 // Null vectors of J, one per pressure basin b with pinned row r_b.  The
 // factored matrix M equals J except in the replaced rows (see apply_pinned).
 // A right null vector n of J with n[r_c] = delta_bc satisfies M n = e_{r_b};
 // a left null vector m of J (m^T J = 0, supported on the continuity rows of
 // basin b, m[r_c] = delta_bc, m[ric] = 0) satisfies M^T m = e_{r_b} - J^T
 // e_{r_b}.  Each is solved with the factors of the given precision, refined
-// twice against FP64 products with M, and kept only if ||J n|| (||J^T m||)
-// is below 1e-10 ||J e_r|| ||n||: a basin whose vector fails that test gets
-// none (its norm is stored as 0).  Computed after each factorisation, on
-// demand.
+// twice against FP64 products with M, and kept if null_check passes; a
+// basin whose vector fails gets none (its norm is stored as 0).  The
+// vectors involve only the continuity rows and the pressure columns of J,
+// which are linear in the state with coefficients fixed by the grid and
+// mask, so they are computed once; after a new Jacobian they are checked
+// again (one product each) and recomputed only if a check fails.
 int rowlu_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count)
 {
     *count = 0;
@@ -812,19 +827,40 @@ int rowlu_null_space(iemic_ctx *c, int precision, bool left, const double **vecs
     const int side = left ? 0 : 1, nb = p->npins;
     const size_t nd = (size_t)c->sc.ndim;
     if (nb == 0 || nb > ROWLU_NULL_MAX) return IEMIC_OK;
-    const unsigned version = precision ? p->f32.version : p->f64.version;
-    if (p->null_prec[side] != precision || p->null_version[side] != version) {
+    if (!p->d_unit) HIPCHK(hipMalloc(&p->d_unit, 3 * nd * sizeof(double)));
+    double *t = p->d_unit, *w1 = p->d_unit + nd, *w2 = p->d_unit + 2 * nd;
+    const dim3 g(nblk(nd, 256)), blk(256);
+    // scale of J near each pinned unknown: ||J e_r|| (||J^T e_r|| when left)
+    auto colnorm = [&](int b, double *out) -> int {
+        hipLaunchKernelGGL(k_unit, g, blk, 0, c->stream, t, (int)nd, p->pin_rows[b]);
+        int r = left ? launch_spmv_t(c, t, w2) : launch_spmv(c, t, w2);
+        if (r) return r;
+        *out = std::sqrt(device_dot(c, w2, w2, (int)nd));
+        return IEMIC_OK;
+    };
+    if (p->d_null[side] && p->null_prec[side] == 1 && precision == 0)
+        for (double v2 : p->null_nrm2[side])
+            if (v2 == 0.0) p->null_prec[side] = -1;   // retry with the FP64 factors
+    if (p->d_null[side] && p->null_prec[side] >= 0 && p->null_version[side] != c->jac_version) {
+        bool all = true;
+        for (int b = 0; b < nb && all; b++) {
+            if (p->null_nrm2[side][b] == 0.0) continue;
+            double cn, vv;
+            bool ok;
+            if ((rc = colnorm(b, &cn))) return rc;
+            if ((rc = null_check(c, left, p->d_null[side] + (size_t)b * nd, cn, w1, &ok, &vv))) return rc;
+            all = ok;
+        }
+        if (all) p->null_version[side] = c->jac_version;
+        else p->null_prec[side] = -1;   // recompute below
+    }
+    if (!p->d_null[side] || p->null_prec[side] < 0) {
         if (!p->d_null[side]) HIPCHK(hipMalloc(&p->d_null[side], (size_t)nb * nd * sizeof(double)));
-        if (!p->d_unit) HIPCHK(hipMalloc(&p->d_unit, 3 * nd * sizeof(double)));
-        double *t = p->d_unit, *w1 = p->d_unit + nd, *w2 = p->d_unit + 2 * nd;
-        const dim3 g(nblk(nd, 256)), blk(256);
         p->null_nrm2[side].assign(nb, 0.0);
         for (int b = 0; b < nb; b++) {
             double *v = p->d_null[side] + (size_t)b * nd;
-            // t = e_r (right) or e_r - J^T e_r (left); w2 = J e_r / J^T e_r
-            hipLaunchKernelGGL(k_unit, g, blk, 0, c->stream, t, (int)nd, p->pin_rows[b]);
-            if ((rc = left ? launch_spmv_t(c, t, w2) : launch_spmv(c, t, w2))) return rc;
-            const double colnorm = std::sqrt(device_dot(c, w2, w2, (int)nd));
+            double cn;
+            if ((rc = colnorm(b, &cn))) return rc;      // leaves t = e_r, w2 = J e_r / J^T e_r
             if (left) hipLaunchKernelGGL(k_sub_from, g, blk, 0, c->stream, t, w2, (int)nd);
             rc = precision ? pinned_solve<float>(c, p, p->f32, left, t, v) : pinned_solve<double>(c, p, p->f64, left, t, v);
             if (rc) return rc;
@@ -836,14 +872,13 @@ int rowlu_null_space(iemic_ctx *c, int precision, bool left, const double **vecs
                 if (rc) return rc;
                 hipLaunchKernelGGL(k_add_to, g, blk, 0, c->stream, v, w2, (int)nd);
             }
-            if ((rc = left ? launch_spmv_t(c, v, w1) : launch_spmv(c, v, w1))) return rc;
-            HIPCHK(hipGetLastError());
-            const double vv = device_dot(c, v, v, (int)nd), res = std::sqrt(device_dot(c, w1, w1, (int)nd));
-            if (std::isnan(vv) || std::isnan(res)) return set_error(IEMIC_EHIP, "null vectors: reduction failed");
-            p->null_nrm2[side][b] = res <= 1e-10 * colnorm * std::sqrt(vv) ? vv : 0.0;
+            double vv;
+            bool ok;
+            if ((rc = null_check(c, left, v, cn, w1, &ok, &vv))) return rc;
+            p->null_nrm2[side][b] = ok ? vv : 0.0;
         }
         p->null_prec[side] = precision;
-        p->null_version[side] = version;
+        p->null_version[side] = c->jac_version;
     }
     *vecs = p->d_null[side];
     *nrm2 = p->null_nrm2[side].data();
