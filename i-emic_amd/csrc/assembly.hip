@@ -866,6 +866,42 @@ __global__ void __launch_bounds__(256) k_spmv(Scalars s, const double *__restric
     for (int e = 0; e < 6; e++) y[6 * (size_t)cell + e] = acc[e];
 }
 
+// Y = J X on nrhs columns (column q at X + q ldx): each thread reads the
+// slots of its cell once for NC columns.  The integral row is added per
+// column afterwards, as in k_spmv.
+template <int NC>
+__global__ void __launch_bounds__(256) k_spmm(Scalars s, const double *__restrict__ J, int nrhs,
+                                              const double *__restrict__ X, size_t ldx, double *__restrict__ Y,
+                                              size_t ldy)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= s.ncell) return;
+    const int q0 = blockIdx.y * NC;
+    const int i = cell % s.n + 1, j = (cell / s.n) % s.m + 1, k = cell / (s.n * s.m) + 1;
+    const size_t C = (size_t)s.ncell;
+    double acc[6][NC];
+#pragma unroll
+    for (int e = 0; e < 6; e++)
+#pragma unroll
+        for (int q = 0; q < NC; q++) acc[e][q] = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; sl++) {
+        int nb;
+        if (nb_cell(s, i, j, k, SLOTS[sl].kk, nb)) {
+            const double a = J[(size_t)sl * C + cell];
+            const size_t col = 6 * (size_t)nb + SLOTS[sl].var;
+#pragma unroll
+            for (int q = 0; q < NC; q++)
+                if (q0 + q < nrhs) acc[SLOTS[sl].eq][q] += a * X[(size_t)(q0 + q) * ldx + col];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; q++)
+        if (q0 + q < nrhs)
+#pragma unroll
+            for (int e = 0; e < 6; e++) Y[(size_t)(q0 + q) * ldy + 6 * (size_t)cell + e] = acc[e][q];
+}
+
 // y = J^T x: each cell gathers the entries of its six columns from the
 // neighbours that reference them; the integral row contributes
 // intSign coef(c) x[rowintcon] to every S column.
@@ -1057,6 +1093,23 @@ int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y)
     HIPCHK(hipGetLastError());
     if (c->sc.rowintcon >= 0)
         return dot_into(c, c->d_coef, d_x, c->sc.ndim, d_y, (double)c->cfg.int_sign, 0.0, c->sc.rowintcon);
+    return IEMIC_OK;
+}
+
+int launch_spmm(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy)
+{
+    constexpr int NC = 4;
+    timed_begin(c, 2);
+    hipLaunchKernelGGL(k_spmm<NC>, dim3(nblk(c->sc.ncell, 256), (nrhs + NC - 1) / NC), dim3(256), 0, c->stream, c->sc,
+                       c->d_J, nrhs, d_X, ldx, d_Y, ldy);
+    timed_end(c, 2);
+    HIPCHK(hipGetLastError());
+    if (c->sc.rowintcon >= 0)
+        for (int q = 0; q < nrhs; q++) {
+            int rc = dot_into(c, c->d_coef, d_X + (size_t)q * ldx, c->sc.ndim, d_Y + (size_t)q * ldy,
+                              (double)c->cfg.int_sign, 0.0, c->sc.rowintcon);
+            if (rc) return rc;
+        }
     return IEMIC_OK;
 }
 

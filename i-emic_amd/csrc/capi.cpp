@@ -21,6 +21,10 @@ int precond_info(iemic_ctx *c, iemic_precond_info *info);
 int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
                  iemic_solve_info *info, bool transpose);
 int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
+int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
+                       iemic_solve_info *infos);
+int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
+                        size_t ldout, float *tmp32);
 void krylov_free(iemic_ctx *c);
 }  // namespace iemic
 
@@ -427,6 +431,51 @@ int iemic_apply_precon_transpose_dev(iemic_ctx *c, const double *d_v, double *d_
     CHECK_CTX(c);
     if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
     return precond_apply_t(c, 0, d_v, d_out);
+}
+
+int iemic_solve_multi_dev(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *d_B, double *d_X,
+                          iemic_solve_info *infos)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    if (nrhs < 1 || !d_B || !d_X) return set_error(IEMIC_EINVAL, "bad right-hand side block");
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return krylov_solve_multi(c, p ? p : &def, nrhs, d_B, d_X, infos);
+}
+
+int iemic_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                      iemic_solve_info *infos)
+{
+    CHECK_CTX(c);
+    if (nrhs < 1 || !B || !X) return set_error(IEMIC_EINVAL, "bad right-hand side block");
+    const size_t bytes = (size_t)nrhs * c->sc.ndim * sizeof(double);
+    double *dB = nullptr, *dX = nullptr;
+    HIPCHK(hipMalloc(&dB, bytes));
+    if (hipMalloc(&dX, bytes) != hipSuccess) {
+        (void)hipFree(dB);
+        return set_error(IEMIC_EHIP, "allocation of the solution block");
+    }
+    int rc = IEMIC_OK;
+    if (hipMemcpy(dB, B, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy B");
+    if (!rc) rc = iemic_solve_multi_dev(c, p, nrhs, dB, dX, infos);
+    if (rc == IEMIC_OK || rc == IEMIC_ENOCONV) {
+        const int keep = rc;
+        if (hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy X");
+        else rc = keep;
+    }
+    (void)hipFree(dB);
+    (void)hipFree(dX);
+    return rc;
+}
+
+int iemic_apply_precon_multi_dev(iemic_ctx *c, int nrhs, const double *d_in, double *d_out)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    if (nrhs < 1 || !d_in || !d_out) return set_error(IEMIC_EINVAL, "bad vector block");
+    const size_t n = c->sc.ndim;
+    return precond_apply_multi(c, 0, nrhs, d_in, n, d_out, n, nullptr);
 }
 
 int iemic_diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad)

@@ -863,6 +863,20 @@ int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out)
 }
 
 int rowlu_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count);
+int rowlu_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out, size_t ldout);
+
+// M^{-1} on nrhs vectors (column q at in + q ldin): one batched sweep for
+// the row LU, a sweep per column for the column ILU
+int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
+                        size_t ldout, float *tmp32)
+{
+    if (c->prec_kind == IEMIC_PREC_ROW_LU) return rowlu_apply_multi(c, precision, nrhs, in, ldin, out, ldout);
+    for (int q = 0; q < nrhs; q++) {
+        int rc = precond_apply(c, precision, in + (size_t)q * ldin, out + (size_t)q * ldout, tmp32);
+        if (rc) return rc;
+    }
+    return IEMIC_OK;
+}
 
 // null vectors of J per pressure basin (row LU only; none for the column ILU)
 int precond_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count)

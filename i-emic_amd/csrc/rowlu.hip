@@ -76,6 +76,8 @@ struct RowLU {
     RowFactors<float> f32;
     int *d_ipiv = nullptr, *d_info = nullptr;
     void *d_v = nullptr, *d_t = nullptr;  // row-layout work vectors (nocean, nmax) in the widest type
+    void *d_vm = nullptr, *d_tm = nullptr; // the same for mcap right-hand sides (multi-vector sweeps)
+    int mcap = 0;
     void *d_w = nullptr;                  // nmax^2 scratch: right-hand sides of the inversion
     rocblas_handle blas = nullptr;
     // null vectors of J, one per pressure basin b (pinned row r_b):
@@ -351,7 +353,7 @@ void rowlu_free(iemic_ctx *c)
     void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_cptr, p->d_cent, p->d_uptr, p->d_uent, p->d_pin,
                     p->d_rs, p->d_cs, p->d_lval, p->d_uval,
                     p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t, p->d_w,
-                    p->d_null[0], p->d_null[1], p->d_unit, p->d_pin_rows};
+                    p->d_null[0], p->d_null[1], p->d_unit, p->d_vm, p->d_tm, p->d_pin_rows};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->blas) rocblas_destroy_handle(p->blas);
@@ -711,6 +713,125 @@ int rowlu_apply(iemic_ctx *c, int precision, const double *in, double *out)
     timed_begin_ext(c, precision ? 6 : 3);
     rc = precision ? rowlu_sweep<float>(c, p->f32, in, out) : rowlu_sweep<double>(c, p->f64, in, out);
     timed_end_ext(c, precision ? 6 : 3);
+    return rc;
+}
+
+// ------------------------------------------------------------------ several right-hand sides
+// Column q of a multi-vector lives at base + q * stride; blockIdx.y is q.
+template <typename T>
+__global__ void k_rowm_gather(int nocean, const int *__restrict__ perm, const double *__restrict__ rs,
+                              const double *__restrict__ in, size_t ldin, T *__restrict__ v)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nocean) return;
+    const size_t q = blockIdx.y;
+    v[q * nocean + u] = (T)(rs[perm[u]] * in[q * ldin + perm[u]]);
+}
+
+template <typename T>
+__global__ void k_rowm_scatter(int nocean, const int *__restrict__ perm, const double *__restrict__ cs,
+                               const T *__restrict__ v, double *__restrict__ out, size_t ldout)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nocean) return;
+    const size_t q = blockIdx.y;
+    out[q * ldout + perm[u]] = cs[perm[u]] * (double)v[q * nocean + u];
+}
+
+__global__ void k_rowm_land(int ncell, const int *__restrict__ loc, const double *__restrict__ in, size_t ldin,
+                            double *__restrict__ out, size_t ldout)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncell || loc[cell] >= 0) return;
+    const size_t q = blockIdx.y;
+#pragma unroll
+    for (int v = 0; v < 6; v++) out[q * ldout + 6 * (size_t)cell + v] = in[q * ldin + 6 * (size_t)cell + v];
+}
+
+// t(:, q) = v_j(:, q) - A_{j,j-1} z_{j-1}(:, q); v columns at stride nocean, t packed (nrow)
+template <typename T>
+__global__ void k_rowm_lmv(int nrow, const int *__restrict__ lptr, const int2 *__restrict__ lent,
+                           const double *__restrict__ lval, const T *__restrict__ vj, const T *__restrict__ zprev,
+                           size_t ldv, T *__restrict__ t)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrow) return;
+    const size_t q = blockIdx.y;
+    const T *zq = zprev + q * ldv;
+    T acc = vj[q * ldv + e];
+    for (int k = lptr[e]; k < lptr[e + 1]; k++) acc -= (T)lval[k] * zq[lent[k].x];
+    t[q * (size_t)nrow + e] = acc;
+}
+
+static rocblas_status gemm(rocblas_handle h, int m, int n, int k, double alpha, const double *A, int lda,
+                           const double *B, int ldb, double beta, double *C, int ldc)
+{
+    return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta, C,
+                         ldc);
+}
+static rocblas_status gemm(rocblas_handle h, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
+                           int ldb, float beta, float *C, int ldc)
+{
+    return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta, C,
+                         ldc);
+}
+
+// M^{-1} on nrhs vectors at once: the same sweep as rowlu_sweep with the
+// dense products as GEMMs (nr x nr times nr x nrhs), so the factors stream
+// from HBM once for all right-hand sides.
+template <typename T>
+static int rowlu_sweep_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, const double *in, size_t ldin,
+                             double *out, size_t ldout)
+{
+    RowLU *p = c->rowlu;
+    if (p->mcap < nrhs) {
+        if (p->d_vm) { (void)hipFree(p->d_vm); (void)hipFree(p->d_tm); p->d_vm = p->d_tm = nullptr; }
+        HIPCHK(hipMalloc(&p->d_vm, (size_t)std::max(p->nocean, 1) * nrhs * sizeof(double)));
+        HIPCHK(hipMalloc(&p->d_tm, (size_t)std::max(p->nmax, 1) * nrhs * sizeof(double)));
+        p->mcap = nrhs;
+    }
+    T *v = reinterpret_cast<T *>(p->d_vm), *t = reinterpret_cast<T *>(p->d_tm);
+    const size_t ldv = (size_t)p->nocean;
+    hipLaunchKernelGGL(k_rowm_land, dim3(nblk(c->sc.ncell, 256), nrhs), dim3(256), 0, c->stream, c->sc.ncell, p->d_loc,
+                       in, ldin, out, ldout);
+    if (p->nocean == 0) return IEMIC_OK;
+    hipLaunchKernelGGL(k_rowm_gather<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean, p->d_perm,
+                       p->d_rs, in, ldin, v);
+    for (int j = 0; j < p->m; j++) {
+        const int nr = p->nj[j];
+        if (nr == 0) continue;
+        T *vj = v + p->ubase[j];
+        const int np = j > 0 ? p->nj[j - 1] : 0;
+        if (np > 0) {
+            hipLaunchKernelGGL(k_rowm_lmv<T>, dim3(nblk(nr, 256), nrhs), dim3(256), 0, c->stream, nr,
+                               p->d_lptr + p->ubase[j], p->d_lent, p->d_lval, vj, v + p->ubase[j - 1], ldv, t);
+        } else {
+            HIPCHK(hipMemcpy2DAsync(t, (size_t)nr * sizeof(T), vj, ldv * sizeof(T), (size_t)nr * sizeof(T), nrhs,
+                                    hipMemcpyDeviceToDevice, c->stream));
+        }
+        BLASCHK(gemm(p->blas, nr, nrhs, nr, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, (int)ldv));
+    }
+    for (int j = p->m - 1; j >= 1; j--) {
+        const int nr = p->nj[j], np = p->nj[j - 1];
+        if (nr == 0 || np == 0) continue;
+        BLASCHK(gemm(p->blas, np, nrhs, nr, (T)-1, F.blk + p->xoff[j], np, v + p->ubase[j], (int)ldv, (T)1,
+                     v + p->ubase[j - 1], (int)ldv));
+    }
+    hipLaunchKernelGGL(k_rowm_scatter<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean,
+                       p->d_perm, p->d_cs, v, out, ldout);
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+
+int rowlu_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out, size_t ldout)
+{
+    int rc = rowlu_ready(c, precision);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    timed_begin_ext(c, 7);
+    rc = precision ? rowlu_sweep_multi<float>(c, p->f32, nrhs, in, ldin, out, ldout)
+                   : rowlu_sweep_multi<double>(c, p->f64, nrhs, in, ldin, out, ldout);
+    timed_end_ext(c, 7);
     return rc;
 }
 

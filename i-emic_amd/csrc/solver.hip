@@ -29,6 +29,8 @@ int precond_apply(iemic_ctx *c, int precision, const double *in, double *out, fl
 int precond_check(iemic_ctx *c);
 int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
 int precond_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count);
+int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
+                        size_t ldout, float *tmp32);
 
 template <typename T>
 struct Basis {
@@ -47,6 +49,10 @@ struct Krylov {
     int hcap = 0;
     int precision = 0;         // preconditioner precision of the current solve
     bool transpose = false;    // solving J^T x = b (adjoint)
+    // several right-hand sides: Krylov blocks j hold the j-th basis vector
+    // of every column, so column q's basis is a strided sequence
+    double *mV = nullptr, *mZ = nullptr, *mW = nullptr, *mB = nullptr, *mh = nullptr;
+    int mm = 0, mq = 0;
 };
 
 // ------------------------------------------------------------------ kernels
@@ -125,6 +131,12 @@ __global__ void k_scale_copy(T *__restrict__ dst, const T *__restrict__ src, dou
     if (r < n) dst[r] = (T)(a * (double)src[r]);
 }
 
+__global__ void k_zero(double *__restrict__ y, int n)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) y[r] = 0.0;
+}
+
 __global__ void k_axpby(double *__restrict__ y, const double *__restrict__ x, double a, double b, int n)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,7 +150,8 @@ void krylov_free(iemic_ctx *c)
 {
     Krylov *k = c->kry;
     if (!k) return;
-    void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->d_h, k->d_part, k->d_r, k->d_bp, k->tmp32};
+    void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->d_h, k->d_part, k->d_r, k->d_bp, k->tmp32,
+                    k->mV, k->mZ, k->mW, k->mB, k->mh};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete k;
@@ -389,6 +402,218 @@ int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, 
     inf.converged = inf.rel_residual <= p->tol;
     if (info) *info = inf;
     return inf.converged ? IEMIC_OK : set_error(IEMIC_ENOCONV, "solver did not reach the tolerance");
+}
+
+// ------------------------------------------------------------------ several right-hand sides
+static int ensure_multi(iemic_ctx *c, int m, int q)
+{
+    Krylov *k = c->kry;
+    if (k->mm >= m && k->mq >= q) return IEMIC_OK;
+    void *bufs[] = {k->mV, k->mZ, k->mW, k->mB, k->mh};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    k->mV = k->mZ = k->mW = k->mB = k->mh = nullptr;
+    k->mm = k->mq = 0;
+    const size_t n = c->sc.ndim;
+    HIPCHK(hipMalloc(&k->mV, (size_t)(m + 1) * q * n * sizeof(double)));
+    HIPCHK(hipMalloc(&k->mZ, (size_t)m * q * n * sizeof(double)));
+    HIPCHK(hipMalloc(&k->mW, (size_t)q * n * sizeof(double)));
+    HIPCHK(hipMalloc(&k->mB, (size_t)q * n * sizeof(double)));
+    HIPCHK(hipMalloc(&k->mh, (size_t)q * 2 * (m + 2) * sizeof(double)));
+    k->mm = m;
+    k->mq = q;
+    return IEMIC_OK;
+}
+
+// per-column 2-norms of a multi-vector (column r at Y + r n)
+static int norms_multi(iemic_ctx *c, int q, const double *Y, std::vector<double> &out)
+{
+    out.resize(q);
+    for (int r = 0; r < q; r++) {
+        int rc = norm2(c, Y + (size_t)r * c->sc.ndim, c->sc.ndim, out[r]);
+        if (rc) return rc;
+    }
+    return IEMIC_OK;
+}
+
+// J X = B for q right-hand sides (column r at B + r ndim), FGMRES per column
+// run in lock step: every iteration applies the preconditioner to the q new
+// basis vectors in one batched sweep and J to them in one SpMM, then
+// orthogonalises each column against its own basis (CGS2) and updates its
+// Givens least-squares problem.  A column that reaches the tolerance is
+// frozen (its later basis vectors are zero); after each cycle the true
+// residuals decide which columns restart.  Like krylov_solve, each column
+// runs on the consistent part of its right-hand side.
+int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
+                       iemic_solve_info *infos)
+{
+    using clk = std::chrono::steady_clock;
+    if (q < 1 || p->restart < 1 || p->maxit < 1 || p->tol <= 0.0 || (p->precision != 0 && p->precision != 1))
+        return set_error(IEMIC_EINVAL, "solver parameters out of range");
+    const int m = p->restart;
+    int rc = ensure_krylov(c, m);
+    if (rc) return rc;
+    if ((rc = ensure_multi(c, m, q))) return rc;
+    Krylov *k = c->kry;
+    k->precision = p->precision;
+    k->transpose = false;
+    const int n = c->sc.ndim;
+    const size_t N = (size_t)n, ldk = (size_t)q * n;   // stride between a column's basis vectors
+    const int nb = nblk(n, 256);
+    auto t0 = clk::now();
+    if ((rc = precond_ready(c, p->precision))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto t1 = clk::now();
+    // consistent right-hand sides into mB; original norms
+    std::vector<double> bnorm;
+    if ((rc = norms_multi(c, q, d_B, bnorm))) return rc;
+    for (int r = 0; r < q; r++) {
+        const double *bc;
+        if ((rc = consistent_rhs(c, p->precision, false, d_B + r * N, &bc))) return rc;
+        HIPCHK(hipMemcpyAsync(k->mB + r * N, bc, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    std::vector<double> bcn;
+    if ((rc = norms_multi(c, q, k->mB, bcn))) return rc;
+    for (int r = 0; r < q; r++) hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, c->stream, d_X + r * N, n);
+    // residual block: starts as the consistent right-hand sides
+    double *R = k->mW;
+    HIPCHK(hipMemcpyAsync(R, k->mB, ldk * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    std::vector<double> beta = bcn;
+    std::vector<int> its(q, 0), cycles(q, 0);
+    std::vector<char> done(q, 0);
+    for (int r = 0; r < q; r++) done[r] = !(bcn[r] > 0.0);
+    std::vector<double> H((size_t)q * (m + 1) * m), cs((size_t)q * (m + 1)), sn((size_t)q * (m + 1)),
+        sv((size_t)q * (m + 1)), hh((size_t)q * 2 * (m + 2)), y(m + 1);
+    std::vector<int> jend(q, 0);
+    std::vector<char> live(q, 0);
+    while (true) {
+        int nlive = 0;
+        for (int r = 0; r < q; r++) {
+            live[r] = !done[r] && its[r] < p->maxit;
+            nlive += live[r];
+            jend[r] = 0;
+            double *v0 = k->mV + r * N;
+            if (live[r]) {
+                hipLaunchKernelGGL(k_scale_copy<double>, dim3(nb), dim3(256), 0, c->stream, v0, R + r * N, 1.0 / beta[r], n);
+                std::fill(sv.begin() + (size_t)r * (m + 1), sv.begin() + (size_t)(r + 1) * (m + 1), 0.0);
+                sv[(size_t)r * (m + 1)] = beta[r];
+            } else {
+                hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, c->stream, v0, n);
+            }
+        }
+        if (nlive == 0) break;
+        int j = 0;
+        for (; j < m; j++) {
+            bool any = false;
+            for (int r = 0; r < q; r++) any |= live[r] && jend[r] == 0;
+            if (!any) break;
+            // Z_j = M^{-1} V_j and W = J Z_j on all q columns
+            double *Vj = k->mV + (size_t)j * ldk, *Zj = k->mZ + (size_t)j * ldk, *W = k->mW;
+            if ((rc = precond_apply_multi(c, p->precision, q, Vj, N, Zj, N, k->tmp32))) return rc;
+            if ((rc = launch_spmm(c, q, Zj, N, W, N))) return rc;
+            for (int r = 0; r < q; r++) {
+                if (!live[r] || jend[r]) continue;
+                const double *Vr = k->mV + r * N;
+                double *w = W + r * N, *dh = k->mh + (size_t)r * 2 * (m + 2), *dh2 = dh + (m + 2);
+                const dim3 gd(k->nblocks, (j + 1 + NV - 1) / NV);
+                hipLaunchKernelGGL(k_multidot<double>, gd, dim3(256), 0, c->stream, Vr, ldk, j + 1, w, n, k->d_part);
+                hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1, dh, 0);
+                hipLaunchKernelGGL(k_update<double>, dim3(nb), dim3(256), 0, c->stream, w, Vr, ldk, j + 1, dh, -1.0, n);
+                hipLaunchKernelGGL(k_multidot<double>, gd, dim3(256), 0, c->stream, Vr, ldk, j + 1, w, n, k->d_part);
+                hipLaunchKernelGGL(k_sum_parts, dim3(j + 1), dim3(256), 0, c->stream, k->d_part, k->nblocks, j + 1, dh2, 0);
+                hipLaunchKernelGGL(k_update<double>, dim3(nb), dim3(256), 0, c->stream, w, Vr, ldk, j + 1, dh2, -1.0, n);
+                hipLaunchKernelGGL(k_add_into, dim3(1), dim3(256), 0, c->stream, dh, dh2, j + 1);
+                hipLaunchKernelGGL(k_multidot<double>, dim3(k->nblocks, 1), dim3(256), 0, c->stream, w, N, 1, w, n,
+                                   k->d_part);
+                hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, k->d_part, k->nblocks, 1, dh + j + 1, 0);
+            }
+            HIPCHK(hipMemcpyAsync(hh.data(), k->mh, (size_t)q * 2 * (m + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int r = 0; r < q; r++) {
+                double *vn = k->mV + (size_t)(j + 1) * ldk + r * N;
+                if (!live[r] || jend[r]) {
+                    hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, c->stream, vn, n);
+                    continue;
+                }
+                double *h = hh.data() + (size_t)r * 2 * (m + 2);
+                double *Hr = H.data() + (size_t)r * (m + 1) * m, *csr = cs.data() + (size_t)r * (m + 1),
+                       *snr = sn.data() + (size_t)r * (m + 1), *s = sv.data() + (size_t)r * (m + 1);
+                const double hn = std::sqrt(h[j + 1]);
+                h[j + 1] = hn;
+                hipLaunchKernelGGL(k_scale_copy<double>, dim3(nb), dim3(256), 0, c->stream, vn, W + r * N,
+                                   hn > 0 ? 1.0 / hn : 0.0, n);
+                for (int t = 0; t < j; t++) {
+                    const double a = csr[t] * h[t] + snr[t] * h[t + 1];
+                    h[t + 1] = -snr[t] * h[t] + csr[t] * h[t + 1];
+                    h[t] = a;
+                }
+                const double den = std::hypot(h[j], h[j + 1]);
+                csr[j] = den == 0.0 ? 1.0 : h[j] / den;
+                snr[j] = den == 0.0 ? 0.0 : h[j + 1] / den;
+                h[j] = csr[j] * h[j] + snr[j] * h[j + 1];
+                h[j + 1] = 0.0;
+                s[j + 1] = -snr[j] * s[j];
+                s[j] = csr[j] * s[j];
+                for (int t = 0; t <= j; t++) Hr[(size_t)t * m + j] = h[t];
+                its[r]++;
+                const double est = std::fabs(s[j + 1]) / bcn[r];
+                if (est < p->tol || hn == 0.0 || its[r] >= p->maxit || j + 1 == m) jend[r] = j + 1;
+            }
+        }
+        // x_r += Z_r y_r, y_r = H_r^{-1} s_r
+        for (int r = 0; r < q; r++) {
+            if (!live[r]) continue;
+            const int je = jend[r] ? jend[r] : j;
+            const double *Hr = H.data() + (size_t)r * (m + 1) * m, *s = sv.data() + (size_t)r * (m + 1);
+            for (int t = je - 1; t >= 0; t--) {
+                double acc = s[t];
+                for (int u = t + 1; u < je; u++) acc -= Hr[(size_t)t * m + u] * y[u];
+                y[t] = acc / Hr[(size_t)t * m + t];
+            }
+            double *dh = k->mh + (size_t)r * 2 * (m + 2);
+            HIPCHK(hipMemcpyAsync(dh, y.data(), je * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(k_update<double>, dim3(nb), dim3(256), 0, c->stream, d_X + r * N, k->mZ + r * N, ldk, je,
+                               dh, 1.0, n);
+            cycles[r]++;
+            // the host copy of y is reused for the next column: wait for this one
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        // true residuals R = B' - J X of the consistent systems
+        if ((rc = launch_spmm(c, q, d_X, N, R, N))) return rc;
+        std::vector<double> one(1, 1.0);
+        double *d_one = k->mh;
+        HIPCHK(hipMemcpyAsync(d_one, one.data(), sizeof(double), hipMemcpyHostToDevice, c->stream));
+        for (int r = 0; r < q; r++) {
+            hipLaunchKernelGGL(k_scale_copy<double>, dim3(nb), dim3(256), 0, c->stream, R + r * N, R + r * N, -1.0, n);
+            hipLaunchKernelGGL(k_update<double>, dim3(nb), dim3(256), 0, c->stream, R + r * N, k->mB + r * N, N, 1,
+                               d_one, 1.0, n);
+        }
+        if ((rc = norms_multi(c, q, R, beta))) return rc;
+        for (int r = 0; r < q; r++)
+            if (live[r] && beta[r] / bcn[r] < p->tol) done[r] = 1;
+    }
+    // reported residuals against the original right-hand sides
+    if ((rc = launch_spmm(c, q, d_X, N, R, N))) return rc;
+    for (int r = 0; r < q; r++)
+        hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, R + r * N, d_B + r * N, 1.0, -1.0, n);
+    std::vector<double> rn;
+    if ((rc = norms_multi(c, q, R, rn))) return rc;
+    const double t_solve = std::chrono::duration<double>(clk::now() - t1).count();
+    if ((rc = precond_check(c))) return rc;
+    bool all = true;
+    for (int r = 0; r < q; r++) {
+        iemic_solve_info inf{};
+        inf.iterations = its[r];
+        inf.restarts = cycles[r];
+        inf.rel_residual = bnorm[r] > 0 ? rn[r] / bnorm[r] : 0.0;
+        inf.t_precon_setup = std::chrono::duration<double>(t1 - t0).count();
+        inf.t_solve = t_solve;
+        inf.converged = inf.rel_residual <= p->tol;
+        all = all && inf.converged;
+        if (infos) infos[r] = inf;
+    }
+    return all ? IEMIC_OK : set_error(IEMIC_ENOCONV, "solver did not reach the tolerance for every right-hand side");
 }
 
 }  // namespace iemic

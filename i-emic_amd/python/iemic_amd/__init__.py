@@ -101,6 +101,9 @@ def load():
         "iemic_apply_matrix_transpose_dev": ([vp, vp, vp], C.c_int),
         "iemic_apply_mass": ([vp, dp, dp], C.c_int),
         "iemic_pressure_projection": ([vp, dp], C.c_int),
+        "iemic_solve_multi": ([vp, vp, C.c_int, dp, dp, vp], C.c_int),
+        "iemic_solve_multi_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
+        "iemic_apply_precon_multi_dev": ([vp, C.c_int, vp, vp], C.c_int),
         "iemic_pressure_projection_dev": ([vp, vp], C.c_int),
         "iemic_set_preconditioner": ([vp, C.c_int], C.c_int),
         "iemic_preconditioner_info": ([vp, _P(PrecondInfo)], C.c_int),
@@ -169,10 +172,19 @@ class DeviceArray:
         _check(load().iemic_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, 8 * self.n))
         return out
 
+    def view(self, start, n):
+        """A non-owning view of elements [start, start + n)."""
+        assert 0 <= start and start + n <= self.n
+        v = DeviceArray.__new__(DeviceArray)
+        v.n = int(n)
+        v.ptr = C.c_void_p(self.ptr.value + 8 * int(start))
+        v._owner = self
+        return v
+
     def free(self):
-        if self.ptr:
+        if self.ptr and not hasattr(self, "_owner"):
             load().iemic_dev_free(self.ptr)
-            self.ptr = C.c_void_p()
+        self.ptr = C.c_void_p()
 
     def __del__(self):
         try:
@@ -377,6 +389,40 @@ class Ocean:
         if rc != 0 and (raise_on_noconv or rc != -6):
             _check(rc)
         return x, info
+
+    def solveMulti(self, B, params=None, raise_on_noconv=True):
+        """J X = B for the columns of B (ndim x nrhs) in one batched solve;
+        returns (X, [SolveInfo per column])."""
+        B = np.asarray(B, dtype=np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        nrhs = B.shape[1]
+        Bc = np.ascontiguousarray(B.T)            # column q contiguous
+        Xc = np.zeros_like(Bc)
+        infos = (SolveInfo * nrhs)()
+        p = params if params is not None else self.solver_params()
+        rc = self.lib.iemic_solve_multi(self.h, C.byref(p), nrhs, _d(Bc), _d(Xc), infos)
+        if rc != 0 and (raise_on_noconv or rc != -6):
+            _check(rc)
+        return Xc.T.copy(), list(infos)
+
+    def solve_multi_dev(self, dB, dX, nrhs, params=None, raise_on_noconv=True):
+        infos = (SolveInfo * nrhs)()
+        p = params if params is not None else self.solver_params()
+        rc = self.lib.iemic_solve_multi_dev(self.h, C.byref(p), nrhs, dB.ptr, dX.ptr, infos)
+        if rc != 0 and (raise_on_noconv or rc != -6):
+            _check(rc)
+        return list(infos)
+
+    def applyPreconMulti(self, V):
+        """M^{-1} applied to the columns of V (ndim x nrhs), one batched sweep."""
+        V = np.asarray(V, dtype=np.float64)
+        nrhs = V.shape[1]
+        dv = DeviceArray(self.ndim * nrhs).upload(np.ascontiguousarray(V.T).reshape(-1))
+        do = DeviceArray(self.ndim * nrhs)
+        _check(self.lib.iemic_apply_precon_multi_dev(self.h, nrhs, dv.ptr, do.ptr))
+        _check(self.lib.iemic_synchronize(self.h))
+        return do.download().reshape(nrhs, self.ndim).T.copy()
 
     def applyPreconTranspose(self, v):
         dv = DeviceArray(self.ndim).upload(v)

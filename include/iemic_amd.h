@@ -172,6 +172,22 @@ int iemic_solve_transpose_dev(iemic_ctx *ctx, const iemic_solver_params *p, cons
                               double *d_x, iemic_solve_info *info);
 int iemic_apply_precon_transpose_dev(iemic_ctx *ctx, const double *d_v, double *d_out);
 
+/* ---- several right-hand sides (no single counterpart in the reference:
+ * Ocean::solve takes one vector, Ocean.C:1070-1147; the Belos block solver
+ * it wraps accepts a multivector) --------------------------------------- */
+/* J X = B for nrhs columns stored one after another (column q at B + q
+ * ndim).  FGMRES per column in lock step; the preconditioner is applied to
+ * all columns in one batched sweep (row LU: the dense products become
+ * GEMMs) and J in one SpMM.  infos (length nrhs, may be NULL) receives each
+ * column's result; returns IEMIC_ENOCONV if any column missed the
+ * tolerance. */
+int iemic_solve_multi(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                      iemic_solve_info *infos);
+int iemic_solve_multi_dev(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *d_B,
+                          double *d_X, iemic_solve_info *infos);
+/* out = M^{-1} in on nrhs columns (FP64 factors) */
+int iemic_apply_precon_multi_dev(iemic_ctx *ctx, int nrhs, const double *d_in, double *d_out);
+
 /* ---- steady-state diagnostics and their state gradients (host arrays) --- */
 /* which = IEMIC_DIAG_MEAN_SST: area-mean surface T (anomaly);
  * which = IEMIC_DIAG_MAX_PSIM: maximum meridional overturning streamfunction,
@@ -191,7 +207,8 @@ int iemic_synchronize(iemic_ctx *ctx);
 /* ---- timing of the last device launches (HIP events, milliseconds) ----- */
 /* kernel: 0 = F assembly, 1 = J assembly, 2 = SpMV, 3 = preconditioner
  * apply (FP64), 4 = preconditioner build (FP64), 5 = build (FP32),
- * 6 = apply (FP32) */
+ * 6 = apply (FP32), 7 = batched apply (several right-hand sides); SpMM
+ * launches are timed under 2 */
 int iemic_last_kernel_ms(const iemic_ctx *ctx, int kernel, double *ms);
 
 #ifdef __cplusplus
