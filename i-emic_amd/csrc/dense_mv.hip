@@ -1,0 +1,135 @@
+// Dense products of the row-LU apply: Y = alpha A X + beta Y with A column-
+// major (m x k, leading dimension lda) and X, Y holding q <= 32 columns.
+// The sweep applies one explicit block inverse or coupling block per zonal
+// row, so the whole apply streams the factors (81 GB at 180x80x24 in FP64)
+// once; these products are HBM-bound for q = 1 and stay close to it for a
+// few tens of right-hand sides.
+//
+// Layout of the work: blocks of 256 rows (one row per thread) times K
+// slices.  Every thread walks the columns of its slice reading A[c lda + r]
+// (consecutive rows of a wavefront read one contiguous 512-byte line per
+// column), with the slice of X staged in LDS 64 columns at a time and read
+// as a broadcast.  Each K slice writes its partial sums to a workspace; a
+// second kernel adds the slices in a fixed order and applies alpha and
+// beta, so the result does not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+namespace iemic {
+
+constexpr int MM_ROWS = 256;   // rows per block, one per thread
+constexpr int MM_KC = 64;      // columns of X staged in LDS per step
+constexpr int MM_BLOCKS = 2048;  // blocks aimed at per product (8 per CU)
+
+template <typename T, int NC>
+__global__ void __launch_bounds__(MM_ROWS) k_mm_partial(int m, int k, int q, int kslice, const T *__restrict__ A,
+                                                        size_t lda, const T *__restrict__ X, size_t ldx,
+                                                        T *__restrict__ P)
+{
+    __shared__ T xs[MM_KC][NC];
+    const int r = blockIdx.x * MM_ROWS + threadIdx.x;
+    const int s = blockIdx.y;
+    const int c0 = s * kslice, c1 = min(k, c0 + kslice);
+    T acc[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) acc[j] = T(0);
+    for (int cb = c0; cb < c1; cb += MM_KC) {
+        const int nc = min(MM_KC, c1 - cb);
+        // X[cb .. cb + nc) of every column: consecutive threads read
+        // consecutive rows of one column of X
+        for (int e = threadIdx.x; e < MM_KC * NC; e += MM_ROWS) {
+            const int j = e / MM_KC, cc = e % MM_KC;
+            xs[cc][j] = (cc < nc && j < q) ? X[(size_t)j * ldx + cb + cc] : T(0);
+        }
+        __syncthreads();
+        if (r < m) {
+            const T *a = A + (size_t)cb * lda + r;
+            int cc = 0;
+            for (; cc + 4 <= nc; cc += 4) {
+                const T a0 = a[(size_t)cc * lda], a1 = a[(size_t)(cc + 1) * lda];
+                const T a2 = a[(size_t)(cc + 2) * lda], a3 = a[(size_t)(cc + 3) * lda];
+#pragma unroll
+                for (int j = 0; j < NC; j++)
+                    acc[j] += a0 * xs[cc][j] + a1 * xs[cc + 1][j] + a2 * xs[cc + 2][j] + a3 * xs[cc + 3][j];
+            }
+            for (; cc < nc; cc++) {
+                const T a0 = a[(size_t)cc * lda];
+#pragma unroll
+                for (int j = 0; j < NC; j++) acc[j] += a0 * xs[cc][j];
+            }
+        }
+        __syncthreads();
+    }
+    if (r < m)
+#pragma unroll
+        for (int j = 0; j < NC; j++)
+            if (j < q) P[((size_t)s * NC + j) * m + r] = acc[j];
+}
+
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_mm_reduce(int m, int q, int ns, const T *__restrict__ P, T alpha, T beta,
+                                                   T *__restrict__ Y, size_t ldy)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (r >= m || j >= q) return;
+    T acc = T(0);
+    for (int s = 0; s < ns; s++) acc += P[((size_t)s * NC + j) * m + r];
+    T *y = Y + (size_t)j * ldy + r;
+    *y = beta == T(0) ? alpha * acc : alpha * acc + beta * *y;
+}
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+template <typename T, int NC>
+static hipError_t mm_launch(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X,
+                            size_t ldx, T beta, T *Y, size_t ldy, T *work, size_t work_elems)
+{
+    const int rb = cdiv(m, MM_ROWS);
+    int ns = 0, kslice = 0;
+    if (k > 0) {
+        ns = std::max(1, std::min(cdiv(MM_BLOCKS, rb), cdiv(k, MM_KC)));
+        while (ns > 1 && (size_t)ns * NC * m > work_elems) ns--;
+        if ((size_t)ns * NC * m > work_elems) return hipErrorInvalidValue;
+        kslice = cdiv(cdiv(k, ns), MM_KC) * MM_KC;
+        ns = cdiv(k, kslice);
+        hipLaunchKernelGGL((k_mm_partial<T, NC>), dim3(rb, ns), dim3(MM_ROWS), 0, st, m, k, q, kslice, A, lda, X, ldx,
+                           work);
+    }
+    hipLaunchKernelGGL((k_mm_reduce<T, NC>), dim3(cdiv(m, 256), q), dim3(256), 0, st, m, q, ns, work, alpha, beta, Y,
+                       ldy);
+    return hipGetLastError();
+}
+
+// Y = alpha A X + beta Y; q columns (column j of X at X + j ldx); work holds
+// at least dense_mm_work(m) elements.  Returns a HIP error code.
+template <typename T>
+hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
+                    T beta, T *Y, size_t ldy, T *work, size_t work_elems)
+{
+    if (m <= 0 || q <= 0) return hipSuccess;
+    for (int j0 = 0; j0 < q; j0 += 32) {
+        const int qq = std::min(32, q - j0);
+        const T *Xj = X + (size_t)j0 * ldx;
+        T *Yj = Y + (size_t)j0 * ldy;
+        hipError_t e;
+        if (qq == 1) e = mm_launch<T, 1>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
+        else if (qq <= 2) e = mm_launch<T, 2>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
+        else if (qq <= 4) e = mm_launch<T, 4>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
+        else if (qq <= 8) e = mm_launch<T, 8>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
+        else if (qq <= 16) e = mm_launch<T, 16>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
+        else e = mm_launch<T, 32>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// elements of workspace dense_mm needs for products with m rows
+size_t dense_mm_work(int m) { return (size_t)32 * 32 * std::max(m, 1); }
+
+template hipError_t dense_mm<double>(hipStream_t, int, int, int, double, const double *, size_t, const double *,
+                                     size_t, double, double *, size_t, double *, size_t);
+template hipError_t dense_mm<float>(hipStream_t, int, int, int, float, const float *, size_t, const float *, size_t,
+                                    float, float *, size_t, float *, size_t);
+
+}  // namespace iemic

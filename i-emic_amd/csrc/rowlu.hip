@@ -46,6 +46,10 @@ namespace iemic {
 std::vector<int> column_tops(const iemic_ctx *c);
 template <typename T>
 int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info);
+template <typename T>
+hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
+                    T beta, T *Y, size_t ldy, T *work, size_t work_elems);
+size_t dense_mm_work(int m);
 std::vector<uint8_t> pressure_pins(const iemic_ctx *c, const std::vector<int> &k0, const std::vector<int> &rank);
 
 template <typename T>
@@ -90,6 +94,8 @@ struct RowLU {
     int null_prec[2] = {-1, -1};
     double *d_unit = nullptr;
     bool dense_lu = false;                // own dense_getrf instead of rocSOLVER getrf (comparison runs)
+    bool blas_apply = false;              // rocBLAS gemv/gemm in the sweeps instead of dense_mm (comparison runs)
+    void *d_mmw = nullptr;                // dense_mm workspace, dense_mm_work(nmax) doubles
 };
 
 static inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
@@ -353,7 +359,7 @@ void rowlu_free(iemic_ctx *c)
     void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_cptr, p->d_cent, p->d_uptr, p->d_uent, p->d_pin,
                     p->d_rs, p->d_cs, p->d_lval, p->d_uval,
                     p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t, p->d_w,
-                    p->d_null[0], p->d_null[1], p->d_unit, p->d_vm, p->d_tm, p->d_pin_rows};
+                    p->d_null[0], p->d_null[1], p->d_unit, p->d_vm, p->d_tm, p->d_mmw, p->d_pin_rows};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->blas) rocblas_destroy_handle(p->blas);
@@ -486,6 +492,8 @@ static int rowlu_ensure(iemic_ctx *c)
     HIPCHK(hipMalloc(&p->d_v, (size_t)std::max(p->nocean, 1) * sizeof(double)));
     HIPCHK(hipMalloc(&p->d_t, (size_t)std::max(p->nmax, 1) * sizeof(double)));
     if (const char *v = std::getenv("IEMIC_DENSE_LU")) p->dense_lu = std::atoi(v) != 0;
+    if (const char *v = std::getenv("IEMIC_BLAS_APPLY")) p->blas_apply = std::atoi(v) != 0;
+    HIPCHK(hipMalloc(&p->d_mmw, dense_mm_work(p->nmax) * sizeof(double)));
     BLASCHK(rocblas_create_handle(&p->blas));
     BLASCHK(rocblas_set_stream(p->blas, c->stream));
     // size the handle's workspace once for the largest row, so that the
@@ -613,6 +621,8 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
 {
     RowLU *p = c->rowlu;
     T *v = reinterpret_cast<T *>(p->d_v), *t = reinterpret_cast<T *>(p->d_t);
+    T *W = reinterpret_cast<T *>(p->d_mmw);
+    const size_t nw = dense_mm_work(p->nmax) * (sizeof(double) / sizeof(T));
     hipLaunchKernelGGL(k_row_land, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc.ncell, p->d_loc, in, out);
     if (p->nocean == 0) return IEMIC_OK;
     hipLaunchKernelGGL(k_row_gather<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
@@ -628,13 +638,18 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
                                p->d_lptr + p->ubase[j], p->d_lent, p->d_lval, vj, v + p->ubase[j - 1], t);
         else
             HIPCHK(hipMemcpyAsync(t, vj, (size_t)nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        BLASCHK(gemv(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
+        if (p->blas_apply) BLASCHK(gemv(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
+        else HIPCHK(dense_mm<T>(c->stream, nr, nr, 1, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, nr, W, nw));
     }
     // backward: z_{j-1} -= X_j z_j
     for (int j = p->m - 1; j >= 1; j--) {
         const int nr = p->nj[j], np = p->nj[j - 1];
         if (nr == 0 || np == 0) continue;
-        BLASCHK(gemv(p->blas, np, nr, (T)-1, F.blk + p->xoff[j], v + p->ubase[j], (T)1, v + p->ubase[j - 1]));
+        if (p->blas_apply)
+            BLASCHK(gemv(p->blas, np, nr, (T)-1, F.blk + p->xoff[j], v + p->ubase[j], (T)1, v + p->ubase[j - 1]));
+        else
+            HIPCHK(dense_mm<T>(c->stream, np, nr, 1, (T)-1, F.blk + p->xoff[j], np, v + p->ubase[j], nr, (T)1,
+                               v + p->ubase[j - 1], np, W, nw));
     }
     hipLaunchKernelGGL(k_row_scatter<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
                        p->d_cs, v, out);
@@ -792,6 +807,8 @@ static int rowlu_sweep_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, con
     }
     T *v = reinterpret_cast<T *>(p->d_vm), *t = reinterpret_cast<T *>(p->d_tm);
     const size_t ldv = (size_t)p->nocean;
+    T *W = reinterpret_cast<T *>(p->d_mmw);
+    const size_t nw = dense_mm_work(p->nmax) * (sizeof(double) / sizeof(T));
     hipLaunchKernelGGL(k_rowm_land, dim3(nblk(c->sc.ncell, 256), nrhs), dim3(256), 0, c->stream, c->sc.ncell, p->d_loc,
                        in, ldin, out, ldout);
     if (p->nocean == 0) return IEMIC_OK;
@@ -809,13 +826,20 @@ static int rowlu_sweep_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, con
             HIPCHK(hipMemcpy2DAsync(t, (size_t)nr * sizeof(T), vj, ldv * sizeof(T), (size_t)nr * sizeof(T), nrhs,
                                     hipMemcpyDeviceToDevice, c->stream));
         }
-        BLASCHK(gemm(p->blas, nr, nrhs, nr, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, (int)ldv));
+        if (p->blas_apply)
+            BLASCHK(gemm(p->blas, nr, nrhs, nr, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, (int)ldv));
+        else
+            HIPCHK(dense_mm<T>(c->stream, nr, nr, nrhs, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, ldv, W, nw));
     }
     for (int j = p->m - 1; j >= 1; j--) {
         const int nr = p->nj[j], np = p->nj[j - 1];
         if (nr == 0 || np == 0) continue;
-        BLASCHK(gemm(p->blas, np, nrhs, nr, (T)-1, F.blk + p->xoff[j], np, v + p->ubase[j], (int)ldv, (T)1,
-                     v + p->ubase[j - 1], (int)ldv));
+        if (p->blas_apply)
+            BLASCHK(gemm(p->blas, np, nrhs, nr, (T)-1, F.blk + p->xoff[j], np, v + p->ubase[j], (int)ldv, (T)1,
+                         v + p->ubase[j - 1], (int)ldv));
+        else
+            HIPCHK(dense_mm<T>(c->stream, np, nr, nrhs, (T)-1, F.blk + p->xoff[j], np, v + p->ubase[j], ldv, (T)1,
+                               v + p->ubase[j - 1], ldv, W, nw));
     }
     hipLaunchKernelGGL(k_rowm_scatter<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean,
                        p->d_perm, p->d_cs, v, out, ldout);
