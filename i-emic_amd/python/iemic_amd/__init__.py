@@ -368,6 +368,9 @@ class Ocean:
     def applyMatrix_dev(self, dv, dout):
         _check(self.lib.iemic_apply_matrix_dev(self.h, dv.ptr, dout.ptr))
 
+    def applyPrecon_dev(self, dv, dout):
+        _check(self.lib.iemic_apply_precon_dev(self.h, dv.ptr, dout.ptr))
+
     def solve_dev(self, db, dx, params=None, raise_on_noconv=True):
         info = SolveInfo()
         p = params if params is not None else self.solver_params()
