@@ -26,6 +26,10 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
 int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
                         size_t ldout, float *tmp32);
 void krylov_free(iemic_ctx *c);
+int newton_dev(iemic_ctx *c, const iemic_solver_params *sp, double tol, double step_tol, int maxit, double *d_x,
+               int *iterations, double *fnorm, int *linear);
+int continuation_dev(iemic_ctx *c, const iemic_solver_params *sp, const iemic_continuation_params *cp, double *d_x,
+                     double *par_hist, double *norm_hist, iemic_continuation_info *out);
 }  // namespace iemic
 
 #define CHECK_CTX(c)                                                            \
@@ -476,6 +480,43 @@ int iemic_apply_precon_multi_dev(iemic_ctx *c, int nrhs, const double *d_in, dou
     if (nrhs < 1 || !d_in || !d_out) return set_error(IEMIC_EINVAL, "bad vector block");
     const size_t n = c->sc.ndim;
     return precond_apply_multi(c, 0, nrhs, d_in, n, d_out, n, nullptr);
+}
+
+int iemic_newton_dev(iemic_ctx *c, const iemic_solver_params *p, double tol, double step_tol, int maxit, double *d_x,
+                     int *iterations, double *fnorm)
+{
+    CHECK_CTX(c);
+    if (!d_x || maxit < 1 || !(tol >= 0.0) || !(step_tol >= 0.0)) return set_error(IEMIC_EINVAL, "bad Newton arguments");
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return newton_dev(c, p ? p : &def, tol, step_tol, maxit, d_x, iterations, fnorm, nullptr);
+}
+
+void iemic_continuation_defaults(iemic_continuation_params *p)
+{
+    if (!p) return;
+    p->par = 19;
+    p->target = 1.0;
+    p->ds = 1e-2;
+    p->ds_min = 1e-8;
+    p->ds_max = 1e3;
+    p->max_steps = 100;
+    p->newton_maxit = 7;
+    p->newton_tol = 1e-4;
+    p->eps = 1e-5;
+    p->tan_scaling = 1.0;
+    p->opt_newton = 3.5;
+    p->predictor_bound = 1e3;
+}
+
+int iemic_continuation_dev(iemic_ctx *c, const iemic_solver_params *p, const iemic_continuation_params *cp,
+                           double *d_x, double *par_hist, double *norm_hist, iemic_continuation_info *info)
+{
+    CHECK_CTX(c);
+    if (!d_x || !cp) return set_error(IEMIC_EINVAL, "null argument");
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return continuation_dev(c, p ? p : &def, cp, d_x, par_hist, norm_hist, info);
 }
 
 int iemic_diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad)

@@ -56,6 +56,19 @@ class SolveInfo(C.Structure):
                 ("t_precon_setup", C.c_double), ("t_solve", C.c_double), ("converged", C.c_int)]
 
 
+class ContinuationParams(C.Structure):
+    _fields_ = [("par", C.c_int), ("target", C.c_double), ("ds", C.c_double), ("ds_min", C.c_double),
+                ("ds_max", C.c_double), ("max_steps", C.c_int), ("newton_maxit", C.c_int),
+                ("newton_tol", C.c_double), ("eps", C.c_double), ("tan_scaling", C.c_double),
+                ("opt_newton", C.c_double), ("predictor_bound", C.c_double)]
+
+
+class ContinuationInfo(C.Structure):
+    _fields_ = [("steps", C.c_int), ("rejected", C.c_int), ("newton_iterations", C.c_int),
+                ("linear_iterations", C.c_int), ("reached", C.c_int), ("par", C.c_double),
+                ("seconds", C.c_double)]
+
+
 PREC_ROW_LU, PREC_COLUMN_ILU = 0, 1
 DIAG_MEAN_SST, DIAG_MAX_PSIM = 0, 1
 
@@ -104,6 +117,9 @@ def load():
         "iemic_solve_multi": ([vp, vp, C.c_int, dp, dp, vp], C.c_int),
         "iemic_solve_multi_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
         "iemic_apply_precon_multi_dev": ([vp, C.c_int, vp, vp], C.c_int),
+        "iemic_newton_dev": ([vp, vp, C.c_double, C.c_double, C.c_int, vp, ip, dp], C.c_int),
+        "iemic_continuation_defaults": ([vp], None),
+        "iemic_continuation_dev": ([vp, vp, vp, vp, dp, dp, vp], C.c_int),
         "iemic_pressure_projection_dev": ([vp, vp], C.c_int),
         "iemic_set_preconditioner": ([vp, C.c_int], C.c_int),
         "iemic_preconditioner_info": ([vp, _P(PrecondInfo)], C.c_int),
@@ -392,6 +408,39 @@ class Ocean:
         if rc != 0 and (raise_on_noconv or rc != -6):
             _check(rc)
         return x, info
+
+    def newton(self, x0, tol=1e-10, step_tol=1e-13, maxit=40, params=None, raise_on_noconv=True):
+        """Damped Newton on device state (iemic_newton_dev); returns
+        (x, iterations, ||F||)."""
+        dx = DeviceArray(self.ndim).upload(x0)
+        its, fn = C.c_int(), C.c_double()
+        p = params if params is not None else self.solver_params(tol=1e-10, restart=50, maxit=200)
+        rc = self.lib.iemic_newton_dev(self.h, C.byref(p), tol, step_tol, maxit, dx.ptr, C.byref(its), C.byref(fn))
+        if rc != 0 and (raise_on_noconv or rc != -6):
+            _check(rc)
+        return dx.download(), its.value, fn.value
+
+    @staticmethod
+    def continuation_params(**kw):
+        p = ContinuationParams()
+        load().iemic_continuation_defaults(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def continuation(self, x0, cparams, params=None):
+        """Pseudo-arclength continuation from the steady state x0 at the
+        current parameter value (iemic_continuation_dev); returns (x,
+        parameter history, ||x|| history, ContinuationInfo)."""
+        dx = DeviceArray(self.ndim).upload(x0)
+        ph = np.full(cparams.max_steps + 1, np.nan)
+        nh = np.full(cparams.max_steps + 1, np.nan)
+        info = ContinuationInfo()
+        p = params if params is not None else self.solver_params(tol=1e-10, restart=50, maxit=200)
+        _check(self.lib.iemic_continuation_dev(self.h, C.byref(p), C.byref(cparams), dx.ptr, _d(ph), _d(nh),
+                                               C.byref(info)))
+        k = info.steps + 1
+        return dx.download(), ph[:k], nh[:k], info
 
     def solveMulti(self, B, params=None, raise_on_noconv=True):
         """J X = B for the columns of B (ndim x nrhs) in one batched solve;

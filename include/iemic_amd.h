@@ -188,6 +188,48 @@ int iemic_solve_multi_dev(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs
 /* out = M^{-1} in on nrhs columns (FP64 factors) */
 int iemic_apply_precon_multi_dev(iemic_ctx *ctx, int nrhs, const double *d_in, double *d_out);
 
+/* ---- callers of the path on device state -------------------------------
+ * Damped Newton on F(x) = 0 (the corrector of Continuation::newtonCorrector,
+ * Continuation.H:611-790, without the arclength row): x -= t dx, J dx = F,
+ * the step's pressure-mode part removed, t halved while ||F|| does not fall.
+ * Stops at ||F|| <= tol or a step with ||t dx||_inf <= step_tol; x (device,
+ * ndim) holds the last iterate.  Returns IEMIC_ENOCONV if neither test was
+ * met within maxit iterations. */
+int iemic_newton_dev(iemic_ctx *ctx, const iemic_solver_params *p, double tol, double step_tol, int maxit,
+                     double *d_x, int *iterations, double *fnorm);
+
+/* Pseudo-arclength continuation in one parameter (Continuation::run / step,
+ * Continuation.H:200-1075): Euler initial tangent, secant tangents, the
+ * quadratic normalisation zeta ||x - x0||^2 + (p - p0)^2 = ds^2 with
+ * zeta = tan_scaling / ndim, a Newton corrector whose two solves per
+ * iteration (J y = dF/dp, J z = -F) are one two-column solve, residual test
+ * max(|dp|, ||dx||_inf) < newton_tol, step control by Newton iterations
+ * (adjustStep) and halving after a failure (reset).  The run ends when the
+ * parameter passes target (a Newton solve at p = target finishes it) or
+ * after max_steps accepted steps.  par_hist / norm_hist (may be NULL) get
+ * the parameter and ||x|| at the start and after every accepted step
+ * (max_steps + 1 entries).  Defaults follow getDefaultInitParameters
+ * (Continuation.H:1330-1385). */
+typedef struct {
+    int par;                 /* parameter index 1..30 (COMB = 19) */
+    double target;
+    double ds, ds_min, ds_max;
+    int max_steps;
+    int newton_maxit;        /* 7 */
+    double newton_tol;       /* 1e-4 */
+    double eps;              /* dF/dp forward-difference step, 1e-5 */
+    double tan_scaling;      /* 1 */
+    double opt_newton;       /* 3.5 */
+    double predictor_bound;  /* 1e3 */
+} iemic_continuation_params;
+typedef struct {
+    int steps, rejected, newton_iterations, linear_iterations, reached;
+    double par, seconds;
+} iemic_continuation_info;
+void iemic_continuation_defaults(iemic_continuation_params *p);
+int iemic_continuation_dev(iemic_ctx *ctx, const iemic_solver_params *p, const iemic_continuation_params *cp,
+                           double *d_x, double *par_hist, double *norm_hist, iemic_continuation_info *info);
+
 /* ---- steady-state diagnostics and their state gradients (host arrays) --- */
 /* which = IEMIC_DIAG_MEAN_SST: area-mean surface T (anomaly);
  * which = IEMIC_DIAG_MAX_PSIM: maximum meridional overturning streamfunction,
