@@ -309,6 +309,14 @@ int iemic_set_preconditioner(iemic_ctx *c, int kind)
     return IEMIC_OK;
 }
 
+int iemic_set_precond_reuse(iemic_ctx *c, int max_age)
+{
+    CHECK_CTX(c);
+    if (max_age < 0) return set_error(IEMIC_EINVAL, "max_age must be >= 0");
+    c->prec_max_age = (unsigned)max_age;
+    return IEMIC_OK;
+}
+
 int iemic_preconditioner_info(iemic_ctx *c, iemic_precond_info *info)
 {
     CHECK_CTX(c);

@@ -61,6 +61,7 @@ struct iemic_ctx {
     bool have_jac = false;
     unsigned jac_version = 0;        // bumped by every Jacobian assembly
     int prec_kind = IEMIC_PREC_ROW_LU;
+    unsigned prec_max_age = 0;       // factors may be this many Jacobians old
     iemic::Precond *prec = nullptr;
     iemic::RowLU *rowlu = nullptr;
     iemic::Krylov *kry = nullptr;

@@ -747,6 +747,8 @@ static int cilu_check(iemic_ctx *c)
     return set_error(IEMIC_EHIP, "preconditioner sweep: dependency wait exceeded its spin limit");
 }
 
+// the column ILU's sweeps read the couplings from J itself, so its factors
+// are always rebuilt for a new Jacobian (reuse applies to the row LU only)
 static bool current(const iemic_ctx *c, bool built, unsigned version)
 {
     return built && version == c->jac_version;
@@ -884,6 +886,14 @@ int precond_null_space(iemic_ctx *c, int precision, bool left, const double **ve
     *count = 0;
     if (c->prec_kind != IEMIC_PREC_ROW_LU) return IEMIC_OK;
     return rowlu_null_space(c, precision, left, vecs, nrm2, count);
+}
+
+int rowlu_stale(iemic_ctx *c, int precision);
+
+// 1 if the current factors were built from an older Jacobian (reuse)
+int precond_stale(iemic_ctx *c, int precision)
+{
+    return c->prec_kind == IEMIC_PREC_ROW_LU ? rowlu_stale(c, precision) : 0;
 }
 
 int precond_check(iemic_ctx *c) { return c->prec_kind == IEMIC_PREC_ROW_LU ? IEMIC_OK : cilu_check(c); }

@@ -695,7 +695,7 @@ static int rowlu_sweep_t(iemic_ctx *c, const RowFactors<T> &F, const double *in,
 
 static bool row_current(const iemic_ctx *c, bool built, unsigned version)
 {
-    return built && version == c->jac_version;
+    return built && c->jac_version - version <= c->prec_max_age;
 }
 
 int rowlu_build(iemic_ctx *c, int precision)
@@ -720,6 +720,16 @@ int rowlu_ready(iemic_ctx *c, int precision)
 
 // M^{-1} in on the model's unknowns (FP64 in and out); the factors and the
 // sweeps run in FP64 (precision 0) or FP32 (precision 1).
+// 1 if the factors of this precision were built from an older Jacobian
+int rowlu_stale(iemic_ctx *c, int precision)
+{
+    RowLU *p = c->rowlu;
+    if (!p) return 0;
+    const RowFactors<double> &F64 = p->f64;
+    return precision ? (p->f32.built && p->f32.version != c->jac_version)
+                     : (F64.built && F64.version != c->jac_version);
+}
+
 int rowlu_apply(iemic_ctx *c, int precision, const double *in, double *out)
 {
     int rc = rowlu_ready(c, precision);

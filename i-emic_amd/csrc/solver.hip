@@ -29,6 +29,8 @@ int precond_apply(iemic_ctx *c, int precision, const double *in, double *out, fl
 int precond_check(iemic_ctx *c);
 int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
 int precond_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count);
+int precond_stale(iemic_ctx *c, int precision);
+int precond_build(iemic_ctx *c, int precision);
 int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
                         size_t ldout, float *tmp32);
 
@@ -396,6 +398,17 @@ int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, 
     inf.iterations = its;
     inf.restarts = cycles;
     if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &inf.rel_residual))) return rc;
+    // factors kept from an older Jacobian (iemic_set_precond_reuse) that did
+    // not reach the tolerance: refactor once and solve again
+    if (inf.rel_residual > p->tol && precond_stale(c, p->precision)) {
+        if ((rc = precond_build(c, p->precision))) return rc;
+        if ((rc = consistent_rhs(c, p->precision, transpose, d_b, &d_bc))) return rc;
+        if ((rc = fgmres<double>(c, d_bc, d_x, p->restart, p->tol, p->maxit, p->verbose, &its, &cycles, &est)))
+            return rc;
+        inf.iterations += its;
+        inf.restarts += cycles;
+        if ((rc = true_residual(c, d_b, d_x, k->d_r, bnorm, &inf.rel_residual))) return rc;
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     inf.t_solve = std::chrono::duration<double>(clk::now() - t1).count();
     if ((rc = precond_check(c))) return rc;
@@ -602,6 +615,14 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
     const double t_solve = std::chrono::duration<double>(clk::now() - t1).count();
     if ((rc = precond_check(c))) return rc;
     bool all = true;
+    for (int r = 0; r < q; r++)
+        if (bnorm[r] > 0 && !(rn[r] / bnorm[r] <= p->tol)) all = false;
+    if (!all && precond_stale(c, p->precision)) {
+        // factors from an older Jacobian missed the tolerance: refactor once
+        if ((rc = precond_build(c, p->precision))) return rc;
+        return krylov_solve_multi(c, p, q, d_B, d_X, infos);
+    }
+    all = true;
     for (int r = 0; r < q; r++) {
         iemic_solve_info inf{};
         inf.iterations = its[r];

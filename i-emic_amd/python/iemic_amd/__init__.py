@@ -119,6 +119,7 @@ def load():
         "iemic_apply_precon_multi_dev": ([vp, C.c_int, vp, vp], C.c_int),
         "iemic_newton_dev": ([vp, vp, C.c_double, C.c_double, C.c_int, vp, ip, dp], C.c_int),
         "iemic_continuation_defaults": ([vp], None),
+        "iemic_set_precond_reuse": ([vp, C.c_int], C.c_int),
         "iemic_continuation_dev": ([vp, vp, vp, vp, dp, dp, vp], C.c_int),
         "iemic_pressure_projection_dev": ([vp, vp], C.c_int),
         "iemic_set_preconditioner": ([vp, C.c_int], C.c_int),
@@ -331,6 +332,10 @@ class Ocean:
         v = np.array(v, dtype=np.float64, copy=True)
         _check(self.lib.iemic_pressure_projection(self.h, _d(v)))
         return v
+
+    def setPreconditionerReuse(self, max_age):
+        """Keep row-LU factors for up to max_age newer Jacobians."""
+        _check(self.lib.iemic_set_precond_reuse(self.h, int(max_age)))
 
     def setPreconditioner(self, kind):
         _check(self.lib.iemic_set_preconditioner(self.h, int(kind)))

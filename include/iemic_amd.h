@@ -146,6 +146,13 @@ int iemic_pressure_projection(iemic_ctx *ctx, double *v);
 int iemic_pressure_projection_dev(iemic_ctx *ctx, double *d_v);
 
 /* ---- Ocean::buildPreconditioner / applyPrecon (Ocean.C:1377-1416) ------ */
+/* reuse of row-LU factors across Jacobians (the reference's Newton-Chord
+ * hybrid, "enable Newton Chord hybrid solve", Continuation.H:1360): the
+ * factors are rebuilt only when they are more than max_age Jacobians old,
+ * or when a solve with older factors misses its tolerance (then once, and
+ * the solve is repeated).  0 (default): rebuilt for every Jacobian. */
+int iemic_set_precond_reuse(iemic_ctx *ctx, int max_age);
+
 /* choose the preconditioner (IEMIC_PREC_*); drops existing factors */
 int iemic_set_preconditioner(iemic_ctx *ctx, int kind);
 int iemic_preconditioner_info(iemic_ctx *ctx, iemic_precond_info *info);

@@ -92,3 +92,25 @@ def test_continuation_reaches_the_target():
     d = oc.pressureProjection(x - xn)
     print(f"|x_arclength - x_natural| {np.linalg.norm(d):.2e} of |x| {np.linalg.norm(x):.3e}")
     assert np.linalg.norm(d) <= 1e-8 * np.linalg.norm(x)
+
+
+def test_factor_reuse_gives_the_same_branch():
+    """Row-LU factors kept over several Jacobians (iemic_set_precond_reuse):
+    the corrector solves still reach their tolerance (stale factors that
+    miss it are rebuilt), so the branch and the end state are the same as
+    with a factorisation per Jacobian."""
+    runs = {}
+    for reuse in (0, 8):
+        oc = gpu_ocean(_smooth("box16"))
+        oc.setPreconditionerReuse(reuse)
+        oc.setPar("COMB", 0.0)
+        cp = oc.continuation_params(par=COMB, target=0.05, ds=1e-2, ds_max=2e-2, max_steps=30, newton_tol=1e-9)
+        runs[reuse] = oc.continuation(np.zeros(oc.ndim), cp)
+        x, ph, nh, info = runs[reuse]
+        print(f"reuse {reuse}: steps {info.steps}, Newton {info.newton_iterations}, "
+              f"linear {info.linear_iterations}, {info.seconds:.2f} s")
+        oc.close()
+    (x0, p0, _, i0), (x8, p8, _, i8) = runs[0], runs[8]
+    assert i0.reached and i8.reached
+    assert np.allclose(p0, p8, rtol=1e-6, atol=1e-9)
+    assert np.linalg.norm(x0 - x8) <= 1e-7 * np.linalg.norm(x0)
