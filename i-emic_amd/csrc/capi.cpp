@@ -534,6 +534,11 @@ int iemic_diagnostic(const iemic_ctx *c, int which, const double *x, double *val
     return diagnostic(c, which, x, value, grad);
 }
 
+int iemic_read_mask(const char *path, int n, int m, int l, int *landm)
+{
+    return read_mask_text(path, n, m, l, landm);
+}
+
 int iemic_dev_alloc(void **ptr, long long bytes)
 {
     if (!ptr || bytes < 0) return set_error(IEMIC_EINVAL, "bad allocation request");

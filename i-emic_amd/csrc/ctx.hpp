@@ -77,6 +77,7 @@ int host_init(iemic_ctx *c, const int *landm, const double *taux, const double *
 void host_parameters_changed(iemic_ctx *c);   // forcing + linear tables (setparcs_)
 void build_graph(iemic_ctx *c);
 int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad);
+int read_mask_text(const char *path, int n, int m, int l, int *landm);
 int set_error(int code, const std::string &msg);
 // assembly.hip
 int upload_tables(iemic_ctx *c);

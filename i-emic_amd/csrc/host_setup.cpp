@@ -623,3 +623,64 @@ int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, do
 }
 
 }  // namespace iemic
+
+namespace iemic {
+
+// The land-mask text files of data/mkmask, as readmask reads them
+// (topo.F90:41-134): for each level k = 0 .. l+1 one header line, then the
+// rows j = m+1 down to 0, each n+2 one-digit values for i = 0 .. n+1
+// (format 362i1: a short record is padded with blanks, which read as 0).
+// Afterwards a land cell above an ocean cell turns the ocean cell below it
+// into land (the land-inversion fix, topo.F90:97-105).  Output: the bordered
+// (n+2)(m+2)(l+2) mask, i fastest.  A file that ends early is an error (the
+// header read of the reference has no error branch).
+int read_mask_text(const char *path, int n, int m, int l, int *landm)
+{
+    if (!path || !landm || n < 1 || m < 1 || l < 1) return set_error(IEMIC_EINVAL, "read_mask: bad arguments");
+    if (n + 2 > 362) return set_error(IEMIC_EUNSUPPORTED, "read_mask: rows longer than 362 values (format 362i1)");
+    std::FILE *f = std::fopen(path, "r");
+    if (!f) return set_error(IEMIC_EINVAL, std::string("read_mask: cannot open ") + path);
+    const int N = n + 2, M = m + 2;
+    auto at = [&](int i, int j, int k) -> int & { return landm[i + N * (j + M * k)]; };
+    std::string line;
+    auto getline = [&](std::string &out) -> bool {
+        out.clear();
+        int ch;
+        bool any = false;
+        while ((ch = std::fgetc(f)) != EOF) {
+            any = true;
+            if (ch == '\n') break;
+            if (ch != '\r') out.push_back((char)ch);
+        }
+        return any;
+    };
+    int rc = IEMIC_OK;
+    for (int k = 0; k <= l + 1 && rc == IEMIC_OK; k++) {
+        if (!getline(line)) rc = set_error(IEMIC_EINVAL, "read_mask: file ends before level " + std::to_string(k));
+        for (int j = m + 1; j >= 0 && rc == IEMIC_OK; j--) {
+            if (!getline(line)) {
+                rc = set_error(IEMIC_EINVAL, "read_mask: file ends in level " + std::to_string(k));
+                break;
+            }
+            for (int i = 0; i < N; i++) {
+                const char ch = i < (int)line.size() ? line[i] : ' ';
+                if (ch == ' ') at(i, j, k) = 0;
+                else if (ch >= '0' && ch <= '9') at(i, j, k) = ch - '0';
+                else {
+                    rc = set_error(IEMIC_EINVAL, "read_mask: not a digit in level " + std::to_string(k) + ", row " +
+                                                     std::to_string(j));
+                    break;
+                }
+            }
+        }
+    }
+    std::fclose(f);
+    if (rc) return rc;
+    for (int i = 1; i <= n; i++)
+        for (int j = 1; j <= m; j++)
+            for (int k = l; k >= 2; k--)
+                if (at(i, j, k) == LAND && at(i, j, k - 1) == OCEAN) at(i, j, k - 1) = LAND;
+    return IEMIC_OK;
+}
+
+}  // namespace iemic

@@ -120,6 +120,7 @@ def load():
         "iemic_newton_dev": ([vp, vp, C.c_double, C.c_double, C.c_int, vp, ip, dp], C.c_int),
         "iemic_continuation_defaults": ([vp], None),
         "iemic_set_precond_reuse": ([vp, C.c_int], C.c_int),
+        "iemic_read_mask": ([C.c_char_p, C.c_int, C.c_int, C.c_int, ip], C.c_int),
         "iemic_continuation_dev": ([vp, vp, vp, vp, dp, dp, vp], C.c_int),
         "iemic_pressure_projection_dev": ([vp, vp], C.c_int),
         "iemic_set_preconditioner": ([vp, C.c_int], C.c_int),
@@ -168,6 +169,13 @@ def _d(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def read_mask(path, n, m, l):
+    """A data/mkmask land mask as iemic_create takes it (iemic_read_mask)."""
+    out = np.zeros((n + 2) * (m + 2) * (l + 2), np.int32)
+    _check(load().iemic_read_mask(str(path).encode(), n, m, l, out.ctypes.data_as(_P(C.c_int))))
+    return out
 
 
 class DeviceArray:

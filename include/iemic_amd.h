@@ -246,6 +246,13 @@ int iemic_continuation_dev(iemic_ctx *ctx, const iemic_solver_params *p, const i
 enum { IEMIC_DIAG_MEAN_SST = 0, IEMIC_DIAG_MAX_PSIM = 1 };
 int iemic_diagnostic(const iemic_ctx *ctx, int which, const double *x, double *value, double *grad);
 
+/* ---- on-disk land masks (readmask, topo.F90:41-134) --------------------- */
+/* Reads a data/mkmask text mask into landm ((n+2)(m+2)(l+2) ints, i fastest,
+ * the layout iemic_create takes), with readmask's land-inversion fix.  Host
+ * only (no GPU needed).  Rows may hold at most 362 values, as the reference
+ * format does. */
+int iemic_read_mask(const char *path, int n, int m, int l, int *landm);
+
 /* ---- device memory helpers for callers without their own allocator ----- */
 int iemic_dev_alloc(void **ptr, long long bytes);
 int iemic_dev_free(void *ptr);

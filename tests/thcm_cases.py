@@ -288,7 +288,8 @@ class RefModel:
 
 def read_mask_file(path, n, m, l):
     """The reference's land-mask text format (topo.F90:50-62): for each level
-    k = 0..l+1 a header line, then rows j = m+1 .. 0 of n+2 digits."""
+    k = 0..l+1 a header line, then rows j = m+1 .. 0 of n+2 digits
+    (Fortran's blank padding of short records read as 0)."""
     landm = np.full((l + 2, m + 2, n + 2), LAND, np.int32)
     with open(path) as f:
         lines = f.read().splitlines()
@@ -299,7 +300,8 @@ def read_mask_file(path, n, m, l):
             row = lines[p]
             p += 1
             for i in range(n + 2):
-                landm[k, j, i] = int(row[i])
+                # a short record pads with blanks, read as 0 (format i1)
+                landm[k, j, i] = int(row[i]) if i < len(row) and row[i] != " " else 0
     # land inversion fix of topo.F90:97-105
     for i in range(1, n + 1):
         for j in range(1, m + 1):
