@@ -6,6 +6,8 @@
 
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "ctx.hpp"
 
@@ -532,6 +534,52 @@ int iemic_diagnostic(const iemic_ctx *c, int which, const double *x, double *val
     CHECK_CTX(c);
     if (!x || !value) return set_error(IEMIC_EINVAL, "null pointer");
     return diagnostic(c, which, x, value, grad);
+}
+
+// One Newton step of one model: F(x), J(x), the preconditioner for J, and
+// J dx = F (x is not changed; x_new = x - dx).
+static int newton_step_one(iemic_ctx *c, const iemic_solver_params *p, const double *d_x, double *d_dx,
+                           iemic_solve_info *info)
+{
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    if ((rc = launch_rhs(c, d_x, c->d_y))) return rc;
+    if ((rc = launch_jacobian(c, d_x))) return rc;
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return krylov_solve(c, p ? p : &def, c->d_y, d_dx, info, false);
+}
+
+int iemic_newton_step_many(iemic_ctx *const *ctx, int count, const double *const *d_x, double *const *d_dx,
+                           const iemic_solver_params *p, iemic_solve_info *infos, int concurrent)
+{
+    if (!ctx || !d_x || !d_dx || count < 1) return set_error(IEMIC_EINVAL, "bad model list");
+    for (int q = 0; q < count; q++)
+        if (!ctx[q] || !d_x[q] || !d_dx[q]) return set_error(IEMIC_EINVAL, "null model or vector");
+    std::vector<int> rcs(count, IEMIC_OK);
+    std::vector<std::string> errs(count);
+    std::vector<iemic_solve_info> inf(count);
+    auto run = [&](int q) {
+        rcs[q] = newton_step_one(ctx[q], p, d_x[q], d_dx[q], &inf[q]);
+        if (rcs[q]) errs[q] = iemic_last_error();   // the message is per thread
+    };
+    if (concurrent) {
+        std::vector<std::thread> th;
+        th.reserve(count);
+        for (int q = 0; q < count; q++) th.emplace_back(run, q);
+        for (auto &t : th) t.join();
+    } else {
+        for (int q = 0; q < count; q++) run(q);
+    }
+    if (infos)
+        for (int q = 0; q < count; q++) infos[q] = inf[q];
+    int first = IEMIC_OK;
+    for (int q = 0; q < count && first == IEMIC_OK; q++)
+        if (rcs[q] && rcs[q] != IEMIC_ENOCONV) first = set_error(rcs[q], "model " + std::to_string(q) + ": " + errs[q]);
+    if (first) return first;
+    for (int q = 0; q < count; q++)
+        if (rcs[q] == IEMIC_ENOCONV) return set_error(IEMIC_ENOCONV, "model " + std::to_string(q) + ": " + errs[q]);
+    return IEMIC_OK;
 }
 
 int iemic_read_mask(const char *path, int n, int m, int l, int *landm)

@@ -121,6 +121,7 @@ def load():
         "iemic_continuation_defaults": ([vp], None),
         "iemic_set_precond_reuse": ([vp, C.c_int], C.c_int),
         "iemic_read_mask": ([C.c_char_p, C.c_int, C.c_int, C.c_int, ip], C.c_int),
+        "iemic_newton_step_many": ([vp, C.c_int, vp, vp, vp, vp, C.c_int], C.c_int),
         "iemic_continuation_dev": ([vp, vp, vp, vp, dp, dp, vp], C.c_int),
         "iemic_pressure_projection_dev": ([vp, vp], C.c_int),
         "iemic_set_preconditioner": ([vp, C.c_int], C.c_int),
@@ -169,6 +170,20 @@ def _d(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def newton_step_many(oceans, dxs, ddxs, params=None, concurrent=True):
+    """One Newton step (F, J, preconditioner, J dx = F) for each model, on
+    one host thread and stream each (iemic_newton_step_many); returns the
+    SolveInfo of each."""
+    q = len(oceans)
+    hs = (C.c_void_p * q)(*[o.h.value for o in oceans])
+    xs = (C.c_void_p * q)(*[d.ptr.value for d in dxs])
+    ys = (C.c_void_p * q)(*[d.ptr.value for d in ddxs])
+    infos = (SolveInfo * q)()
+    p = params if params is not None else oceans[0].solver_params()
+    _check(load().iemic_newton_step_many(hs, q, xs, ys, C.byref(p), infos, int(bool(concurrent))))
+    return list(infos)
 
 
 def read_mask(path, n, m, l):

@@ -246,6 +246,17 @@ int iemic_continuation_dev(iemic_ctx *ctx, const iemic_solver_params *p, const i
 enum { IEMIC_DIAG_MEAN_SST = 0, IEMIC_DIAG_MAX_PSIM = 1 };
 int iemic_diagnostic(const iemic_ctx *ctx, int which, const double *x, double *value, double *grad);
 
+/* ---- several independent models (no counterpart in the reference: one
+ * THCM instance per process, THCM.H:76-84) ----------------------------------
+ * One Newton step per context: F(x_q), J(x_q), the preconditioner and the
+ * solve J dx_q = F(x_q), with every context on its own host thread and its
+ * own stream when concurrent != 0 (the pivoted factorisation of one model
+ * leaves most of the GPU idle; several models fill it), else one after the
+ * other.  Contexts must be distinct.  infos (length count, may be NULL) get
+ * each solve's result. */
+int iemic_newton_step_many(iemic_ctx *const *ctx, int count, const double *const *d_x, double *const *d_dx,
+                           const iemic_solver_params *p, iemic_solve_info *infos, int concurrent);
+
 /* ---- on-disk land masks (readmask, topo.F90:41-134) --------------------- */
 /* Reads a data/mkmask text mask into landm ((n+2)(m+2)(l+2) ints, i fastest,
  * the layout iemic_create takes), with readmask's land-inversion fix.  Host

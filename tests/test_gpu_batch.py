@@ -86,3 +86,40 @@ def test_multi_solve_with_the_column_ilu():
         rel = np.linalg.norm(B[:, q] - J @ X[:, q]) / np.linalg.norm(B[:, q])
         print(f"column {q}: {infos[q].iterations} iterations, residual {rel:.2e}")
         assert rel <= 1e-8
+
+
+def _basin(n, m, l):
+    from iemic_amd import basins
+    cfg_d, lm, frc, x0, pars = basins.make_global(n, m, l)
+    cfg = iemic_amd.Config(**{f: cfg_d[f] for f, _ in iemic_amd.Config._fields_})
+    oc = iemic_amd.Ocean(cfg, lm, frc["taux"], frc["tauy"], frc["tatm"], frc["emip"], frc["spert"])
+    for k, v in pars.items():
+        oc.setPar(k, v)
+    return oc, x0
+
+
+def test_independent_basins_concurrently_and_one_by_one():
+    """Newton steps of four seeded basins of different sizes, one host thread
+    and stream each, against the same steps one after the other: the same
+    solutions (to rounding: the factorisations may reduce in another order
+    under contention) and every solve at the FP64 bar."""
+    sizes = [(24, 12, 6), (32, 16, 8), (16, 16, 4), (40, 20, 6)]
+    models = [_basin(*s) for s in sizes]
+    oceans = [oc for oc, _ in models]
+    dxs = [iemic_amd.DeviceArray(oc.ndim).upload(x0) for oc, x0 in models]
+    out = {}
+    for conc in (False, True):
+        ddx = [iemic_amd.DeviceArray(oc.ndim) for oc in oceans]
+        infos = iemic_amd.newton_step_many(oceans, dxs, ddx, oceans[0].solver_params(tol=1e-10, restart=50, maxit=200),
+                                           concurrent=conc)
+        out[conc] = [d.download() for d in ddx]
+        print(("concurrent" if conc else "sequential"), [(i.iterations, f"{i.rel_residual:.1e}") for i in infos])
+        assert all(i.converged for i in infos)
+    for q, (oc, x0) in enumerate(models):
+        a, b = out[False][q], out[True][q]
+        print(f"basin {sizes[q]}: max difference {np.abs(a - b).max() / np.abs(a).max():.2e}")
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
+        F = oc.computeRHS(x0)
+        oc.computeJacobian(x0)
+        J = _csr(oc)
+        assert np.linalg.norm(F - J @ b) <= 1e-8 * np.linalg.norm(F)
