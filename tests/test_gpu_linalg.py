@@ -201,3 +201,30 @@ def test_column_ilu_dataflow_sweep_matches_level_launches():
         finally:
             del os.environ["IEMIC_LEVEL_LAUNCH"]
     assert np.array_equal(out["0"], out["1"])
+
+
+def test_solve_with_an_all_land_row_and_two_basins():
+    """A 16x16x8 box cut by a zonal land wall (row j = 8 all land): a row
+    block with no ocean unknowns, and two closed basins whose pressure modes
+    each need a pin.  Salinity restoring (sres = 1): with a salt flux each
+    closed basin would carry its own mean-salinity null mode, which the one
+    integral condition cannot fix (as in the reference)."""
+    from iemic_amd import basins
+    from thcm_cases import Case, Config, seeded_state as _seed
+    cfg_d, lm, frc, x0, pars = basins.make_box(16, 16, 8)
+    cfg_d = dict(cfg_d)
+    cfg_d["sres"] = 1
+    lm = lm.reshape(10, 18, 18).copy()
+    lm[:, 8, :] = 1
+    cfg = Config(**{f: cfg_d[f] for f, _ in Config._fields_})
+    case = Case("wall", cfg, lm.reshape(-1), frc, pars, _seed(cfg, lm.reshape(-1), 11))
+    oc = gpu_ocean(case)
+    F = oc.computeRHS(case.x)
+    oc.computeJacobian(case.x)
+    y, info = oc.solve(F, oc.solver_params(tol=1e-8, restart=50, maxit=200), raise_on_noconv=False)
+    rel = np.linalg.norm(F - _csr(oc) @ y) / np.linalg.norm(F)
+    pinfo = oc.preconditionerInfo()
+    print(f"wall box: pins {pinfo.pins}, rows {pinfo.blocks}, {info.iterations} iterations, residual {rel:.2e}")
+    assert pinfo.pins >= 4
+    assert rel <= 1e-8
+    assert info.iterations <= 30
