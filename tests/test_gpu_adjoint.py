@@ -183,3 +183,39 @@ def test_sensitivity_matches_oracle_jacobian(natl8_steady):
         print(f"{par}: gpu {sens[par]:.12e} oracle {ref:.12e} (term scale {scale[par]:.3e})")
         # vanishing sensitivities: both sides are rounding, below 1e-12 |g| / |p|
         assert abs(sens[par] - ref) <= 1e-8 * abs(ref) + 1e-12 * (scale[par] + abs(g0) / abs(p0))
+
+
+def test_torch_autograd_wrapper_gives_the_adjoint_gradient(natl8_steady, fd_table):
+    """The steady-state diagnostic as a torch.autograd.Function of the four
+    parameters (test harness only): forward = Newton to the steady state on
+    the device model, backward = the one-solve adjoint.  p.grad must match
+    the extrapolated finite differences of steady states."""
+    torch = pytest.importorskip("torch")
+    case, oc, xs = natl8_steady
+    which = iemic_amd.DIAG_MAX_PSIM
+
+    class SteadyDiagnostic(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, p):
+            old = [oc.getPar(k) for k in PARS]
+            for k, v in zip(PARS, p.tolist()):
+                oc.setPar(k, v)
+            x, _ = adjoint.newton(oc, xs, tol=1e-14)
+            g, sens, _, _ = adjoint.sensitivity(oc, x, which, PARS)
+            for k, v in zip(PARS, old):
+                oc.setPar(k, v)
+            ctx.save_for_backward(torch.tensor([sens[k] for k in PARS], dtype=torch.float64))
+            return torch.tensor(g, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (dg,) = ctx.saved_tensors
+            return grad_out * dg
+
+    p = torch.tensor([oc.getPar(k) for k in PARS], dtype=torch.float64, requires_grad=True)
+    g = SteadyDiagnostic.apply(p)
+    g.backward()
+    for i, par in enumerate(PARS):
+        _, _, fd, h = fd_table[which, par]
+        print(f"{par}: autograd {p.grad[i].item():.12e} fd {fd:.12e}")
+        assert abs(p.grad[i].item() - fd) <= 1e-6 * abs(fd) + 1e-12 * abs(g.item()) / h
