@@ -28,6 +28,7 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
 int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
                         size_t ldout, float *tmp32);
 void krylov_free(iemic_ctx *c);
+int rowlu_tuned_blocks(iemic_ctx *c, int nrhs);
 int newton_dev(iemic_ctx *c, const iemic_solver_params *sp, double tol, double step_tol, int maxit, double *d_x,
                int *iterations, double *fnorm, int *linear);
 int continuation_dev(iemic_ctx *c, const iemic_solver_params *sp, const iemic_continuation_params *cp, double *d_x,
@@ -308,6 +309,14 @@ int iemic_set_preconditioner(iemic_ctx *c, int kind)
     HIPCHK(hipStreamSynchronize(c->stream));
     precond_free(c);
     c->prec_kind = kind;
+    return IEMIC_OK;
+}
+
+int iemic_apply_tuning(iemic_ctx *c, int nrhs, int *blocks)
+{
+    CHECK_CTX(c);
+    if (!blocks) return set_error(IEMIC_EINVAL, "null pointer");
+    *blocks = c->prec_kind == IEMIC_PREC_ROW_LU ? rowlu_tuned_blocks(c, nrhs) : 0;
     return IEMIC_OK;
 }
 

@@ -20,7 +20,7 @@ namespace iemic {
 
 constexpr int MM_ROWS = 256;   // rows per block, one per thread
 constexpr int MM_KC = 64;      // columns of X staged in LDS per step
-constexpr int MM_BLOCKS = 2048;  // blocks aimed at per product (8 per CU)
+constexpr int MM_BLOCKS = 2048;  // default blocks aimed at per product (8 per CU); tuned by the caller
 
 template <typename T, int NC>
 __global__ void __launch_bounds__(MM_ROWS) k_mm_partial(int m, int k, int q, int kslice, const T *__restrict__ A,
@@ -83,12 +83,12 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 template <typename T, int NC>
 static hipError_t mm_launch(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X,
-                            size_t ldx, T beta, T *Y, size_t ldy, T *work, size_t work_elems)
+                            size_t ldx, T beta, T *Y, size_t ldy, T *work, size_t work_elems, int blocks)
 {
     const int rb = cdiv(m, MM_ROWS);
     int ns = 0, kslice = 0;
     if (k > 0) {
-        ns = std::max(1, std::min(cdiv(MM_BLOCKS, rb), cdiv(k, MM_KC)));
+        ns = std::max(1, std::min(cdiv(blocks > 0 ? blocks : MM_BLOCKS, rb), cdiv(k, MM_KC)));
         while (ns > 1 && (size_t)ns * NC * m > work_elems) ns--;
         if ((size_t)ns * NC * m > work_elems) return hipErrorInvalidValue;
         kslice = cdiv(cdiv(k, ns), MM_KC) * MM_KC;
@@ -102,10 +102,12 @@ static hipError_t mm_launch(hipStream_t st, int m, int k, int q, T alpha, const 
 }
 
 // Y = alpha A X + beta Y; q columns (column j of X at X + j ldx); work holds
-// at least dense_mm_work(m) elements.  Returns a HIP error code.
+// at least dense_mm_work(m) elements; blocks: the number of blocks (row
+// blocks times K slices) to aim at, 0 for the default.  Returns a HIP error
+// code.
 template <typename T>
 hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
-                    T beta, T *Y, size_t ldy, T *work, size_t work_elems)
+                    T beta, T *Y, size_t ldy, T *work, size_t work_elems, int blocks)
 {
     if (m <= 0 || q <= 0) return hipSuccess;
     for (int j0 = 0; j0 < q; j0 += 32) {
@@ -113,12 +115,12 @@ hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, si
         const T *Xj = X + (size_t)j0 * ldx;
         T *Yj = Y + (size_t)j0 * ldy;
         hipError_t e;
-        if (qq == 1) e = mm_launch<T, 1>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
-        else if (qq <= 2) e = mm_launch<T, 2>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
-        else if (qq <= 4) e = mm_launch<T, 4>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
-        else if (qq <= 8) e = mm_launch<T, 8>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
-        else if (qq <= 16) e = mm_launch<T, 16>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
-        else e = mm_launch<T, 32>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems);
+        if (qq == 1) e = mm_launch<T, 1>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
+        else if (qq <= 2) e = mm_launch<T, 2>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
+        else if (qq <= 4) e = mm_launch<T, 4>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
+        else if (qq <= 8) e = mm_launch<T, 8>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
+        else if (qq <= 16) e = mm_launch<T, 16>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
+        else e = mm_launch<T, 32>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -128,8 +130,8 @@ hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, si
 size_t dense_mm_work(int m) { return (size_t)32 * 32 * std::max(m, 1); }
 
 template hipError_t dense_mm<double>(hipStream_t, int, int, int, double, const double *, size_t, const double *,
-                                     size_t, double, double *, size_t, double *, size_t);
+                                     size_t, double, double *, size_t, double *, size_t, int);
 template hipError_t dense_mm<float>(hipStream_t, int, int, int, float, const float *, size_t, const float *, size_t,
-                                    float, float *, size_t, float *, size_t);
+                                    float, float *, size_t, float *, size_t, int);
 
 }  // namespace iemic

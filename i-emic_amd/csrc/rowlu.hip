@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <vector>
 
 #include "ctx.hpp"
@@ -48,7 +49,7 @@ template <typename T>
 int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info);
 template <typename T>
 hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
-                    T beta, T *Y, size_t ldy, T *work, size_t work_elems);
+                    T beta, T *Y, size_t ldy, T *work, size_t work_elems, int blocks);
 size_t dense_mm_work(int m);
 std::vector<uint8_t> pressure_pins(const iemic_ctx *c, const std::vector<int> &k0, const std::vector<int> &rank);
 
@@ -96,6 +97,7 @@ struct RowLU {
     bool dense_lu = false;                // own dense_getrf instead of rocSOLVER getrf (comparison runs)
     bool blas_apply = false;              // rocBLAS gemv/gemm in the sweeps instead of dense_mm (comparison runs)
     void *d_mmw = nullptr;                // dense_mm workspace, dense_mm_work(nmax) doubles
+    std::map<int, int> mm_blocks;         // tuned dense_mm block target per number of right-hand sides
 };
 
 static inline int nblk(size_t n, int b) { return (int)((n + b - 1) / b); }
@@ -617,7 +619,7 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
 }
 
 template <typename T>
-static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, double *out)
+static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, double *out, int blocks = 0)
 {
     RowLU *p = c->rowlu;
     T *v = reinterpret_cast<T *>(p->d_v), *t = reinterpret_cast<T *>(p->d_t);
@@ -639,7 +641,7 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
         else
             HIPCHK(hipMemcpyAsync(t, vj, (size_t)nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
         if (p->blas_apply) BLASCHK(gemv(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
-        else HIPCHK(dense_mm<T>(c->stream, nr, nr, 1, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, nr, W, nw));
+        else HIPCHK(dense_mm<T>(c->stream, nr, nr, 1, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, nr, W, nw, blocks));
     }
     // backward: z_{j-1} -= X_j z_j
     for (int j = p->m - 1; j >= 1; j--) {
@@ -649,7 +651,7 @@ static int rowlu_sweep(iemic_ctx *c, const RowFactors<T> &F, const double *in, d
             BLASCHK(gemv(p->blas, np, nr, (T)-1, F.blk + p->xoff[j], v + p->ubase[j], (T)1, v + p->ubase[j - 1]));
         else
             HIPCHK(dense_mm<T>(c->stream, np, nr, 1, (T)-1, F.blk + p->xoff[j], np, v + p->ubase[j], nr, (T)1,
-                               v + p->ubase[j - 1], np, W, nw));
+                               v + p->ubase[j - 1], np, W, nw, blocks));
     }
     hipLaunchKernelGGL(k_row_scatter<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
                        p->d_cs, v, out);
@@ -730,13 +732,71 @@ int rowlu_stale(iemic_ctx *c, int precision)
                      : (F64.built && F64.version != c->jac_version);
 }
 
+// Autotuning of the dense products' launch shape: the number of blocks
+// (row blocks times K slices) each product aims at decides how far a long
+// factor row is split; fewer slices read the right-hand sides less often,
+// more keep more CUs streaming.  The first apply with a given number of
+// right-hand sides times every candidate on the real operands (the sweep
+// is a pure function of its input, so each run simply rewrites the output)
+// and keeps the fastest for this context.
+static const int MM_CANDIDATES[] = {512, 1024, 2048, 4096, 8192};
+
+template <typename Run>
+static int tune_blocks(iemic_ctx *c, int q, Run run)
+{
+    RowLU *p = c->rowlu;
+    auto it = p->mm_blocks.find(q);
+    if (it != p->mm_blocks.end()) return it->second;
+    if (std::getenv("IEMIC_NO_TUNE")) return p->mm_blocks[q] = 2048;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 2048;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return 2048;
+    }
+    int best = 2048;
+    float tbest = 1e30f;
+    for (int cand : MM_CANDIDATES) {
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            (void)hipEventRecord(e0, c->stream);
+            if (run(cand)) break;
+            (void)hipEventRecord(e1, c->stream);
+            float ms = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+            tmin = std::min(tmin, ms);
+        }
+        if (tmin < tbest) {
+            tbest = tmin;
+            best = cand;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGetLastError();
+    return p->mm_blocks[q] = best;
+}
+
+int rowlu_tuned_blocks(iemic_ctx *c, int nrhs)
+{
+    if (!c->rowlu) return 0;
+    auto it = c->rowlu->mm_blocks.find(nrhs);
+    return it == c->rowlu->mm_blocks.end() ? 0 : it->second;
+}
+
 int rowlu_apply(iemic_ctx *c, int precision, const double *in, double *out)
 {
     int rc = rowlu_ready(c, precision);
     if (rc) return rc;
     RowLU *p = c->rowlu;
+    if (!p->blas_apply && in != out)   // tuning rewrites out, so not in place
+        (void)tune_blocks(c, precision ? -1 : 1, [&](int b) {
+            return precision ? rowlu_sweep<float>(c, p->f32, in, out, b) : rowlu_sweep<double>(c, p->f64, in, out, b);
+        });
+    auto tb = p->mm_blocks.find(precision ? -1 : 1);
+    const int blocks = p->blas_apply || tb == p->mm_blocks.end() ? 0 : tb->second;
     timed_begin_ext(c, precision ? 6 : 3);
-    rc = precision ? rowlu_sweep<float>(c, p->f32, in, out) : rowlu_sweep<double>(c, p->f64, in, out);
+    rc = precision ? rowlu_sweep<float>(c, p->f32, in, out, blocks) : rowlu_sweep<double>(c, p->f64, in, out, blocks);
     timed_end_ext(c, precision ? 6 : 3);
     return rc;
 }
@@ -806,7 +866,7 @@ static rocblas_status gemm(rocblas_handle h, int m, int n, int k, float alpha, c
 // from HBM once for all right-hand sides.
 template <typename T>
 static int rowlu_sweep_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, const double *in, size_t ldin,
-                             double *out, size_t ldout)
+                             double *out, size_t ldout, int blocks = 0)
 {
     RowLU *p = c->rowlu;
     if (p->mcap < nrhs) {
@@ -839,7 +899,8 @@ static int rowlu_sweep_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, con
         if (p->blas_apply)
             BLASCHK(gemm(p->blas, nr, nrhs, nr, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, (int)ldv));
         else
-            HIPCHK(dense_mm<T>(c->stream, nr, nr, nrhs, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, ldv, W, nw));
+            HIPCHK(dense_mm<T>(c->stream, nr, nr, nrhs, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, ldv, W, nw,
+                               blocks));
     }
     for (int j = p->m - 1; j >= 1; j--) {
         const int nr = p->nj[j], np = p->nj[j - 1];
@@ -849,7 +910,7 @@ static int rowlu_sweep_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, con
                          v + p->ubase[j - 1], (int)ldv));
         else
             HIPCHK(dense_mm<T>(c->stream, np, nr, nrhs, (T)-1, F.blk + p->xoff[j], np, v + p->ubase[j], ldv, (T)1,
-                               v + p->ubase[j - 1], ldv, W, nw));
+                               v + p->ubase[j - 1], ldv, W, nw, blocks));
     }
     hipLaunchKernelGGL(k_rowm_scatter<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean,
                        p->d_perm, p->d_cs, v, out, ldout);
@@ -862,9 +923,17 @@ int rowlu_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, s
     int rc = rowlu_ready(c, precision);
     if (rc) return rc;
     RowLU *p = c->rowlu;
+    const int key = precision ? -nrhs : nrhs;
+    if (!p->blas_apply && in != out)
+        (void)tune_blocks(c, key, [&](int b) {
+            return precision ? rowlu_sweep_multi<float>(c, p->f32, nrhs, in, ldin, out, ldout, b)
+                             : rowlu_sweep_multi<double>(c, p->f64, nrhs, in, ldin, out, ldout, b);
+        });
+    auto tb = p->mm_blocks.find(key);
+    const int blocks = p->blas_apply || tb == p->mm_blocks.end() ? 0 : tb->second;
     timed_begin_ext(c, 7);
-    rc = precision ? rowlu_sweep_multi<float>(c, p->f32, nrhs, in, ldin, out, ldout)
-                   : rowlu_sweep_multi<double>(c, p->f64, nrhs, in, ldin, out, ldout);
+    rc = precision ? rowlu_sweep_multi<float>(c, p->f32, nrhs, in, ldin, out, ldout, blocks)
+                   : rowlu_sweep_multi<double>(c, p->f64, nrhs, in, ldin, out, ldout, blocks);
     timed_end_ext(c, 7);
     return rc;
 }

@@ -120,6 +120,7 @@ def load():
         "iemic_newton_dev": ([vp, vp, C.c_double, C.c_double, C.c_int, vp, ip, dp], C.c_int),
         "iemic_continuation_defaults": ([vp], None),
         "iemic_set_precond_reuse": ([vp, C.c_int], C.c_int),
+        "iemic_apply_tuning": ([vp, C.c_int, ip], C.c_int),
         "iemic_read_mask": ([C.c_char_p, C.c_int, C.c_int, C.c_int, ip], C.c_int),
         "iemic_newton_step_many": ([vp, C.c_int, vp, vp, vp, vp, C.c_int], C.c_int),
         "iemic_continuation_dev": ([vp, vp, vp, vp, dp, dp, vp], C.c_int),
@@ -355,6 +356,12 @@ class Ocean:
         v = np.array(v, dtype=np.float64, copy=True)
         _check(self.lib.iemic_pressure_projection(self.h, _d(v)))
         return v
+
+    def applyTuning(self, nrhs=1):
+        """Block target the apply tuned for nrhs right-hand sides (0: none)."""
+        b = C.c_int()
+        _check(self.lib.iemic_apply_tuning(self.h, int(nrhs), C.byref(b)))
+        return b.value
 
     def setPreconditionerReuse(self, max_age):
         """Keep row-LU factors for up to max_age newer Jacobians."""

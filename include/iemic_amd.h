@@ -146,6 +146,12 @@ int iemic_pressure_projection(iemic_ctx *ctx, double *v);
 int iemic_pressure_projection_dev(iemic_ctx *ctx, double *d_v);
 
 /* ---- Ocean::buildPreconditioner / applyPrecon (Ocean.C:1377-1416) ------ */
+/* launch shape the row-LU apply tuned for nrhs right-hand sides (nrhs = 1:
+ * the single apply; negative: FP32 factors): the number of blocks its dense
+ * products aim at, chosen by timing candidates on the first apply; 0 if
+ * not tuned yet (IEMIC_NO_TUNE in the environment keeps the default). */
+int iemic_apply_tuning(iemic_ctx *ctx, int nrhs, int *blocks);
+
 /* reuse of row-LU factors across Jacobians (the reference's Newton-Chord
  * hybrid, "enable Newton Chord hybrid solve", Continuation.H:1360): the
  * factors are rebuilt only when they are more than max_age Jacobians old,

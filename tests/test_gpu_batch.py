@@ -123,3 +123,31 @@ def test_independent_basins_concurrently_and_one_by_one():
         oc.computeJacobian(x0)
         J = _csr(oc)
         assert np.linalg.norm(F - J @ b) <= 1e-8 * np.linalg.norm(F)
+
+
+def test_apply_tuning_keeps_the_operator():
+    """The first apply times the candidate launch shapes of the dense
+    products and keeps one; later applies reuse it bit for bit, and the
+    operator is the rocBLAS-applied one to rounding (the K split changes
+    only the order of summation)."""
+    import os
+    case = _case("gateway16")
+    v = np.random.default_rng(8).normal(size=case.ndim)
+    oc = gpu_ocean(case)
+    oc.computeJacobian(case.x)
+    z1 = oc.applyPrecon(v)
+    tuned = oc.applyTuning(1)
+    z2 = oc.applyPrecon(v)
+    oc.close()
+    os.environ["IEMIC_BLAS_APPLY"] = "1"
+    try:
+        ob = gpu_ocean(case)
+        ob.computeJacobian(case.x)
+        zb = ob.applyPrecon(v)
+        ob.close()
+    finally:
+        del os.environ["IEMIC_BLAS_APPLY"]
+    print(f"tuned block target {tuned}; against rocBLAS {np.abs(z1 - zb).max() / np.abs(zb).max():.2e}")
+    assert tuned in (512, 1024, 2048, 4096, 8192)
+    assert np.array_equal(z1, z2)
+    assert np.abs(z1 - zb).max() <= 1e-8 * np.abs(zb).max()
