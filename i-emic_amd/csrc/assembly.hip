@@ -1113,6 +1113,16 @@ int launch_spmm(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d
     return IEMIC_OK;
 }
 
+// Y = J^T X on nrhs columns, one transposed product per column
+int launch_spmm_t(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy)
+{
+    for (int q = 0; q < nrhs; q++) {
+        int rc = launch_spmv_t(c, d_X + (size_t)q * ldx, d_Y + (size_t)q * ldy);
+        if (rc) return rc;
+    }
+    return IEMIC_OK;
+}
+
 int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y)
 {
     hipLaunchKernelGGL(k_spmv_t, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, c->sc, c->d_J,

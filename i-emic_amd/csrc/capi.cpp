@@ -24,7 +24,7 @@ int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, 
                  iemic_solve_info *info, bool transpose);
 int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
 int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
-                       iemic_solve_info *infos);
+                       iemic_solve_info *infos, bool transpose);
 int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
                         size_t ldout, float *tmp32);
 void krylov_free(iemic_ctx *c);
@@ -464,11 +464,22 @@ int iemic_solve_multi_dev(iemic_ctx *c, const iemic_solver_params *p, int nrhs, 
     if (nrhs < 1 || !d_B || !d_X) return set_error(IEMIC_EINVAL, "bad right-hand side block");
     iemic_solver_params def;
     iemic_solver_defaults(&def);
-    return krylov_solve_multi(c, p ? p : &def, nrhs, d_B, d_X, infos);
+    return krylov_solve_multi(c, p ? p : &def, nrhs, d_B, d_X, infos, false);
 }
 
-int iemic_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
-                      iemic_solve_info *infos)
+int iemic_solve_multi_transpose_dev(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *d_B,
+                                    double *d_X, iemic_solve_info *infos)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    if (nrhs < 1 || !d_B || !d_X) return set_error(IEMIC_EINVAL, "bad right-hand side block");
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return krylov_solve_multi(c, p ? p : &def, nrhs, d_B, d_X, infos, true);
+}
+
+static int solve_multi_host(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                            iemic_solve_info *infos, bool transpose)
 {
     CHECK_CTX(c);
     if (nrhs < 1 || !B || !X) return set_error(IEMIC_EINVAL, "bad right-hand side block");
@@ -481,7 +492,9 @@ int iemic_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int nrhs, cons
     }
     int rc = IEMIC_OK;
     if (hipMemcpy(dB, B, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy B");
-    if (!rc) rc = iemic_solve_multi_dev(c, p, nrhs, dB, dX, infos);
+    if (!rc)
+        rc = transpose ? iemic_solve_multi_transpose_dev(c, p, nrhs, dB, dX, infos)
+                       : iemic_solve_multi_dev(c, p, nrhs, dB, dX, infos);
     if (rc == IEMIC_OK || rc == IEMIC_ENOCONV) {
         const int keep = rc;
         if (hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy X");
@@ -490,6 +503,18 @@ int iemic_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int nrhs, cons
     (void)hipFree(dB);
     (void)hipFree(dX);
     return rc;
+}
+
+int iemic_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                      iemic_solve_info *infos)
+{
+    return solve_multi_host(c, p, nrhs, B, X, infos, false);
+}
+
+int iemic_solve_multi_transpose(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                                iemic_solve_info *infos)
+{
+    return solve_multi_host(c, p, nrhs, B, X, infos, true);
 }
 
 int iemic_apply_precon_multi_dev(iemic_ctx *c, int nrhs, const double *d_in, double *d_out)

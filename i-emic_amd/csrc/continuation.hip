@@ -20,7 +20,7 @@ namespace iemic {
 int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x, iemic_solve_info *info,
                  bool transpose);
 int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
-                       iemic_solve_info *infos);
+                       iemic_solve_info *infos, bool transpose);
 
 namespace {
 
@@ -251,7 +251,7 @@ int continuation_dev(iemic_ctx *c, const iemic_solver_params *sp, const iemic_co
             const double rbp = ds * ds - zeta * w.dot(diff, diff) - pd * pd;
             if ((rc = launch_jacobian(c, d_x))) return rc;
             iemic_solve_info infos[2];
-            rc = krylov_solve_multi(c, sp, 2, FB, YZ, infos);
+            rc = krylov_solve_multi(c, sp, 2, FB, YZ, infos, false);
             if (rc && rc != IEMIC_ENOCONV) return rc;
             linear_total += infos[0].iterations + infos[1].iterations;
             if ((rc = launch_pressure_projection(c, y)) || (rc = launch_pressure_projection(c, z))) return rc;

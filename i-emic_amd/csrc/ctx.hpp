@@ -86,6 +86,7 @@ int launch_jacobian(iemic_ctx *c, const double *d_x);
 int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_spmm(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy);
+int launch_spmm_t(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy);
 int launch_export(iemic_ctx *c, double *d_val);
 int launch_pressure_projection(iemic_ctx *c, double *d_v);
 double device_dot(iemic_ctx *c, const double *d_a, const double *d_b, int n);

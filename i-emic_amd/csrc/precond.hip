@@ -866,6 +866,17 @@ int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out)
 
 int rowlu_null_space(iemic_ctx *c, int precision, bool left, const double **vecs, const double **nrm2, int *count);
 int rowlu_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out, size_t ldout);
+int rowlu_apply_multi_t(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
+                        size_t ldout);
+
+// M^{-T} on nrhs vectors (row LU only, as the single transposed apply)
+int precond_apply_multi_t(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
+                          size_t ldout)
+{
+    if (c->prec_kind != IEMIC_PREC_ROW_LU)
+        return set_error(IEMIC_EUNSUPPORTED, "transposed preconditioner: row LU only");
+    return rowlu_apply_multi_t(c, precision, nrhs, in, ldin, out, ldout);
+}
 
 // M^{-1} on nrhs vectors (column q at in + q ldin): one batched sweep for
 // the row LU, a sweep per column for the column ILU

@@ -918,6 +918,87 @@ static int rowlu_sweep_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, con
     return IEMIC_OK;
 }
 
+// t(:, q) = y_j(:, q) - A_{j+1,j}^T x_{j+1}(:, q); y, x at stride ldv, t packed (nrow)
+template <typename T>
+__global__ void k_rowm_ltmv(int nrow, const int *__restrict__ cptr, const int2 *__restrict__ cent,
+                            const double *__restrict__ lval, const T *__restrict__ yj, const T *__restrict__ xnext,
+                            size_t ldv, T *__restrict__ t)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrow) return;
+    const size_t q = blockIdx.y;
+    const T *xq = xnext + q * ldv;
+    T acc = yj[q * ldv + e];
+    for (int k = cptr[e]; k < cptr[e + 1]; k++) acc -= (T)lval[cent[k].y] * xq[cent[k].x];
+    t[q * (size_t)nrow + e] = acc;
+}
+
+static rocblas_status gemm_t(rocblas_handle h, int m, int n, int k, double alpha, const double *A, int lda,
+                             const double *B, int ldb, double beta, double *C, int ldc)
+{
+    return rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta,
+                         C, ldc);
+}
+static rocblas_status gemm_t(rocblas_handle h, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
+                             int ldb, float beta, float *C, int ldc)
+{
+    return rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta,
+                         C, ldc);
+}
+
+// M^{-T} on nrhs vectors: rowlu_sweep_t with GEMMs over all columns
+template <typename T>
+static int rowlu_sweep_t_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, const double *in, size_t ldin,
+                               double *out, size_t ldout)
+{
+    RowLU *p = c->rowlu;
+    if (p->mcap < nrhs) {
+        if (p->d_vm) { (void)hipFree(p->d_vm); (void)hipFree(p->d_tm); p->d_vm = p->d_tm = nullptr; }
+        HIPCHK(hipMalloc(&p->d_vm, (size_t)std::max(p->nocean, 1) * nrhs * sizeof(double)));
+        HIPCHK(hipMalloc(&p->d_tm, (size_t)std::max(p->nmax, 1) * nrhs * sizeof(double)));
+        p->mcap = nrhs;
+    }
+    T *v = reinterpret_cast<T *>(p->d_vm), *t = reinterpret_cast<T *>(p->d_tm);
+    const size_t ldv = (size_t)p->nocean;
+    hipLaunchKernelGGL(k_rowm_land, dim3(nblk(c->sc.ncell, 256), nrhs), dim3(256), 0, c->stream, c->sc.ncell, p->d_loc,
+                       in, ldin, out, ldout);
+    if (p->nocean == 0) return IEMIC_OK;
+    hipLaunchKernelGGL(k_rowm_gather<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean, p->d_perm,
+                       p->d_cs, in, ldin, v);
+    for (int j = 0; j + 1 < p->m; j++) {
+        const int nr = p->nj[j], nn = p->nj[j + 1];
+        if (nr == 0 || nn == 0) continue;
+        BLASCHK(gemm_t(p->blas, nn, nrhs, nr, (T)-1, F.blk + p->xoff[j + 1], nr, v + p->ubase[j], (int)ldv, (T)1,
+                       v + p->ubase[j + 1], (int)ldv));
+    }
+    for (int j = p->m - 1; j >= 0; j--) {
+        const int nr = p->nj[j];
+        if (nr == 0) continue;
+        T *vj = v + p->ubase[j];
+        if (j + 1 < p->m && p->nj[j + 1] > 0)
+            hipLaunchKernelGGL(k_rowm_ltmv<T>, dim3(nblk(nr, 256), nrhs), dim3(256), 0, c->stream, nr,
+                               p->d_cptr + p->ubase[j], p->d_cent, p->d_lval, vj, v + p->ubase[j + 1], ldv, t);
+        else
+            HIPCHK(hipMemcpy2DAsync(t, (size_t)nr * sizeof(T), vj, ldv * sizeof(T), (size_t)nr * sizeof(T), nrhs,
+                                    hipMemcpyDeviceToDevice, c->stream));
+        BLASCHK(gemm_t(p->blas, nr, nrhs, nr, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, (int)ldv));
+    }
+    hipLaunchKernelGGL(k_rowm_scatter<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean,
+                       p->d_perm, p->d_rs, v, out, ldout);
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+
+int rowlu_apply_multi_t(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
+                        size_t ldout)
+{
+    int rc = rowlu_ready(c, precision);
+    if (rc) return rc;
+    RowLU *p = c->rowlu;
+    return precision ? rowlu_sweep_t_multi<float>(c, p->f32, nrhs, in, ldin, out, ldout)
+                     : rowlu_sweep_t_multi<double>(c, p->f64, nrhs, in, ldin, out, ldout);
+}
+
 int rowlu_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out, size_t ldout)
 {
     int rc = rowlu_ready(c, precision);

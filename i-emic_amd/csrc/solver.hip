@@ -33,6 +33,8 @@ int precond_stale(iemic_ctx *c, int precision);
 int precond_build(iemic_ctx *c, int precision);
 int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
                         size_t ldout, float *tmp32);
+int precond_apply_multi_t(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
+                          size_t ldout);
 
 template <typename T>
 struct Basis {
@@ -458,7 +460,7 @@ static int norms_multi(iemic_ctx *c, int q, const double *Y, std::vector<double>
 // residuals decide which columns restart.  Like krylov_solve, each column
 // runs on the consistent part of its right-hand side.
 int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
-                       iemic_solve_info *infos)
+                       iemic_solve_info *infos, bool transpose)
 {
     using clk = std::chrono::steady_clock;
     if (q < 1 || p->restart < 1 || p->maxit < 1 || p->tol <= 0.0 || (p->precision != 0 && p->precision != 1))
@@ -469,9 +471,16 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
     if ((rc = ensure_multi(c, m, q))) return rc;
     Krylov *k = c->kry;
     k->precision = p->precision;
-    k->transpose = false;
+    k->transpose = transpose;
     const int n = c->sc.ndim;
-    const size_t N = (size_t)n, ldk = (size_t)q * n;   // stride between a column's basis vectors
+    const size_t N = (size_t)n, ldk = (size_t)q * n;
+    auto prec = [&](const double *in, double *out) {
+        return transpose ? precond_apply_multi_t(c, p->precision, q, in, N, out, N)
+                         : precond_apply_multi(c, p->precision, q, in, N, out, N, k->tmp32);
+    };
+    auto mat = [&](const double *in, double *out) {
+        return transpose ? launch_spmm_t(c, q, in, N, out, N) : launch_spmm(c, q, in, N, out, N);
+    };   // stride between a column's basis vectors
     const int nb = nblk(n, 256);
     auto t0 = clk::now();
     if ((rc = precond_ready(c, p->precision))) return rc;
@@ -482,7 +491,7 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
     if ((rc = norms_multi(c, q, d_B, bnorm))) return rc;
     for (int r = 0; r < q; r++) {
         const double *bc;
-        if ((rc = consistent_rhs(c, p->precision, false, d_B + r * N, &bc))) return rc;
+        if ((rc = consistent_rhs(c, p->precision, transpose, d_B + r * N, &bc))) return rc;
         HIPCHK(hipMemcpyAsync(k->mB + r * N, bc, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
     std::vector<double> bcn;
@@ -522,8 +531,8 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
             if (!any) break;
             // Z_j = M^{-1} V_j and W = J Z_j on all q columns
             double *Vj = k->mV + (size_t)j * ldk, *Zj = k->mZ + (size_t)j * ldk, *W = k->mW;
-            if ((rc = precond_apply_multi(c, p->precision, q, Vj, N, Zj, N, k->tmp32))) return rc;
-            if ((rc = launch_spmm(c, q, Zj, N, W, N))) return rc;
+            if ((rc = prec(Vj, Zj))) return rc;
+            if ((rc = mat(Zj, W))) return rc;
             for (int r = 0; r < q; r++) {
                 if (!live[r] || jend[r]) continue;
                 const double *Vr = k->mV + r * N;
@@ -593,7 +602,7 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
             HIPCHK(hipStreamSynchronize(c->stream));
         }
         // true residuals R = B' - J X of the consistent systems
-        if ((rc = launch_spmm(c, q, d_X, N, R, N))) return rc;
+        if ((rc = mat(d_X, R))) return rc;
         std::vector<double> one(1, 1.0);
         double *d_one = k->mh;
         HIPCHK(hipMemcpyAsync(d_one, one.data(), sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -607,7 +616,7 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
             if (live[r] && beta[r] / bcn[r] < p->tol) done[r] = 1;
     }
     // reported residuals against the original right-hand sides
-    if ((rc = launch_spmm(c, q, d_X, N, R, N))) return rc;
+    if ((rc = mat(d_X, R))) return rc;
     for (int r = 0; r < q; r++)
         hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, R + r * N, d_B + r * N, 1.0, -1.0, n);
     std::vector<double> rn;
@@ -620,7 +629,7 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
     if (!all && precond_stale(c, p->precision)) {
         // factors from an older Jacobian missed the tolerance: refactor once
         if ((rc = precond_build(c, p->precision))) return rc;
-        return krylov_solve_multi(c, p, q, d_B, d_X, infos);
+        return krylov_solve_multi(c, p, q, d_B, d_X, infos, transpose);
     }
     all = true;
     for (int r = 0; r < q; r++) {

@@ -116,6 +116,8 @@ def load():
         "iemic_pressure_projection": ([vp, dp], C.c_int),
         "iemic_solve_multi": ([vp, vp, C.c_int, dp, dp, vp], C.c_int),
         "iemic_solve_multi_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
+        "iemic_solve_multi_transpose": ([vp, vp, C.c_int, dp, dp, vp], C.c_int),
+        "iemic_solve_multi_transpose_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
         "iemic_apply_precon_multi_dev": ([vp, C.c_int, vp, vp], C.c_int),
         "iemic_newton_dev": ([vp, vp, C.c_double, C.c_double, C.c_int, vp, ip, dp], C.c_int),
         "iemic_continuation_defaults": ([vp], None),
@@ -477,9 +479,9 @@ class Ocean:
         k = info.steps + 1
         return dx.download(), ph[:k], nh[:k], info
 
-    def solveMulti(self, B, params=None, raise_on_noconv=True):
-        """J X = B for the columns of B (ndim x nrhs) in one batched solve;
-        returns (X, [SolveInfo per column])."""
+    def solveMulti(self, B, params=None, raise_on_noconv=True, transpose=False):
+        """J X = B (J^T X = B if transpose) for the columns of B (ndim x
+        nrhs) in one batched solve; returns (X, [SolveInfo per column])."""
         B = np.asarray(B, dtype=np.float64)
         if B.ndim == 1:
             B = B[:, None]
@@ -488,7 +490,8 @@ class Ocean:
         Xc = np.zeros_like(Bc)
         infos = (SolveInfo * nrhs)()
         p = params if params is not None else self.solver_params()
-        rc = self.lib.iemic_solve_multi(self.h, C.byref(p), nrhs, _d(Bc), _d(Xc), infos)
+        fn = self.lib.iemic_solve_multi_transpose if transpose else self.lib.iemic_solve_multi
+        rc = fn(self.h, C.byref(p), nrhs, _d(Bc), _d(Xc), infos)
         if rc != 0 and (raise_on_noconv or rc != -6):
             _check(rc)
         return Xc.T.copy(), list(infos)

@@ -198,6 +198,12 @@ int iemic_solve_multi(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, co
                       iemic_solve_info *infos);
 int iemic_solve_multi_dev(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *d_B,
                           double *d_X, iemic_solve_info *infos);
+/* J^T X = B for nrhs columns (adjoint batches: several diagnostics or
+ * states), the same lock-step FGMRES with the transposed batched sweep */
+int iemic_solve_multi_transpose(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                                iemic_solve_info *infos);
+int iemic_solve_multi_transpose_dev(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *d_B,
+                                    double *d_X, iemic_solve_info *infos);
 /* out = M^{-1} in on nrhs columns (FP64 factors) */
 int iemic_apply_precon_multi_dev(iemic_ctx *ctx, int nrhs, const double *d_in, double *d_out);
 

@@ -151,3 +151,28 @@ def test_apply_tuning_keeps_the_operator():
     assert tuned in (512, 1024, 2048, 4096, 8192)
     assert np.array_equal(z1, z2)
     assert np.abs(z1 - zb).max() <= 1e-8 * np.abs(zb).max()
+
+
+@pytest.mark.parametrize("name", ["natl8", "gateway16", "global96"])
+def test_adjoint_batch(name):
+    """J^T X = G for an adjoint batch: the gradients of the two steady-state
+    diagnostics and three J^T y columns, every column to the FP64 bar and
+    equal to single adjoint solves (to the solve tolerance)."""
+    case = _case(name)
+    oc = gpu_ocean(case)
+    oc.computeJacobian(case.x)
+    J = _csr(oc)
+    rng = np.random.default_rng(17)
+    cols = [oc.diagnostic(iemic_amd.DIAG_MEAN_SST, case.x)[1], oc.diagnostic(iemic_amd.DIAG_MAX_PSIM, case.x)[1]]
+    cols += [J.T @ rng.normal(size=oc.ndim) for _ in range(3)]
+    G = np.stack(cols, axis=1)
+    p = oc.solver_params(tol=1e-10, restart=20, maxit=200)
+    L, infos = oc.solveMulti(G, p, transpose=True)
+    for q in range(G.shape[1]):
+        rel = np.linalg.norm(G[:, q] - J.T @ L[:, q]) / np.linalg.norm(G[:, q])
+        lam, _ = oc.solveTranspose(G[:, q], p)
+        # both solutions of a singular system: compare their products with J^T
+        d = np.linalg.norm(J.T @ (L[:, q] - lam)) / np.linalg.norm(G[:, q])
+        print(f"{name} column {q}: {infos[q].iterations} iterations, residual {rel:.2e}, vs single {d:.2e}")
+        assert rel <= 1e-8
+        assert d <= 1e-8
