@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <type_traits>
 
 namespace iemic {
 
@@ -79,7 +81,91 @@ __global__ void __launch_bounds__(256) k_mm_reduce(int m, int q, int ns, const T
     *y = beta == T(0) ? alpha * acc : alpha * acc + beta * *y;
 }
 
+// FP64 matrix-core variant for 9..32 right-hand sides (CT column tiles of
+// 16), v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16) with lane l
+// holding A[l % 16][l / 16], B[l / 16][l % 16] and the four accumulator
+// values v = 0..3 of D[l / 16 + 4 v][l % 16].  A wavefront owns 64 rows (four
+// 16-row tiles) of its block's 256; each step of 4 columns loads one factor
+// element per tile and lane (a column's 64 rows are one contiguous 512-byte
+// line across the four loads) and takes the B tiles from the LDS slice of X.
+typedef double mm_d4 __attribute__((ext_vector_type(4)));
+
+template <int CT>
+__global__ void __launch_bounds__(MM_ROWS) k_mm_partial_mfma(int m, int k, int q, int kslice,
+                                                             const double *__restrict__ A, size_t lda,
+                                                             const double *__restrict__ X, size_t ldx,
+                                                             double *__restrict__ P)
+{
+    constexpr int NC = 16 * CT;
+    __shared__ double xs[MM_KC][NC + 1];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int row0 = blockIdx.x * MM_ROWS + wave * 64;
+    const int s = blockIdx.y;
+    const int c0 = s * kslice, c1 = min(k, c0 + kslice);
+    const int lr = lane % 16, lk = lane / 16;
+    mm_d4 acc[4][CT];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int u = 0; u < CT; u++) acc[t][u] = mm_d4{0.0, 0.0, 0.0, 0.0};
+    for (int cb = c0; cb < c1; cb += MM_KC) {
+        const int nc = min(MM_KC, c1 - cb);
+        for (int e = threadIdx.x; e < MM_KC * NC; e += MM_ROWS) {
+            const int j = e / MM_KC, cc = e % MM_KC;
+            xs[cc][j] = (cc < nc && j < q) ? X[(size_t)j * ldx + cb + cc] : 0.0;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < nc; kk += 4) {
+            const int kc = kk + lk;   // < MM_KC; rows of xs at or beyond nc are zero
+            double a[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int r = row0 + 16 * t + lr;
+                a[t] = (kc < nc && r < m) ? A[(size_t)(cb + kc) * lda + r] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < CT; u++) {
+                const double b = xs[kc][16 * u + lr];
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, acc[t][u], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int u = 0; u < CT; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = row0 + 16 * t + lk + 4 * v, c = 16 * u + lr;
+                if (r < m && c < q) P[((size_t)s * NC + c) * m + r] = acc[t][u][v];
+            }
+}
+
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+template <int CT>
+static hipError_t mm_launch_mfma(hipStream_t st, int m, int k, int q, double alpha, const double *A, size_t lda,
+                                 const double *X, size_t ldx, double beta, double *Y, size_t ldy, double *work,
+                                 size_t work_elems, int blocks)
+{
+    constexpr int NC = 16 * CT;
+    const int rb = cdiv(m, MM_ROWS);
+    int ns = 0, kslice = 0;
+    if (k > 0) {
+        ns = std::max(1, std::min(cdiv(blocks > 0 ? blocks : MM_BLOCKS, rb), cdiv(k, MM_KC)));
+        while (ns > 1 && (size_t)ns * NC * m > work_elems) ns--;
+        if ((size_t)ns * NC * m > work_elems) return hipErrorInvalidValue;
+        kslice = cdiv(cdiv(k, ns), MM_KC) * MM_KC;
+        ns = cdiv(k, kslice);
+        hipLaunchKernelGGL((k_mm_partial_mfma<CT>), dim3(rb, ns), dim3(MM_ROWS), 0, st, m, k, q, kslice, A, lda, X,
+                           ldx, work);
+    }
+    hipLaunchKernelGGL((k_mm_reduce<double, NC>), dim3(cdiv(m, 256), q), dim3(256), 0, st, m, q, ns, work, alpha,
+                       beta, Y, ldy);
+    return hipGetLastError();
+}
 
 template <typename T, int NC>
 static hipError_t mm_launch(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X,
@@ -101,6 +187,14 @@ static hipError_t mm_launch(hipStream_t st, int m, int k, int q, T alpha, const 
     return hipGetLastError();
 }
 
+// FP64 products with 9..32 right-hand sides run on the matrix cores unless
+// IEMIC_NO_MFMA is set (comparison runs)
+static bool mm_use_mfma()
+{
+    static const bool use = std::getenv("IEMIC_NO_MFMA") == nullptr;
+    return use;
+}
+
 // Y = alpha A X + beta Y; q columns (column j of X at X + j ldx); work holds
 // at least dense_mm_work(m) elements; blocks: the number of blocks (row
 // blocks times K slices) to aim at, 0 for the default.  Returns a HIP error
@@ -115,6 +209,14 @@ hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, si
         const T *Xj = X + (size_t)j0 * ldx;
         T *Yj = Y + (size_t)j0 * ldy;
         hipError_t e;
+        if constexpr (std::is_same<T, double>::value) {
+            if (qq > 8 && mm_use_mfma()) {
+                e = qq > 16 ? mm_launch_mfma<2>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks)
+                            : mm_launch_mfma<1>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
+                if (e != hipSuccess) return e;
+                continue;
+            }
+        }
         if (qq == 1) e = mm_launch<T, 1>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
         else if (qq <= 2) e = mm_launch<T, 2>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
         else if (qq <= 4) e = mm_launch<T, 4>(st, m, k, qq, alpha, A, lda, Xj, ldx, beta, Yj, ldy, work, work_elems, blocks);
