@@ -558,6 +558,8 @@ int iemic_continuation_dev(iemic_ctx *c, const iemic_solver_params *p, const iem
 {
     CHECK_CTX(c);
     if (!d_x || !cp) return set_error(IEMIC_EINVAL, "null argument");
+    if (cp->par == 6 || cp->par == 29)   // MIXP, MKAP: as iemic_set_par
+        return set_error(IEMIC_EUNSUPPORTED, "neutral physics (MIXP/MKAP) is not supported");
     iemic_solver_params def;
     iemic_solver_defaults(&def);
     return continuation_dev(c, p ? p : &def, cp, d_x, par_hist, norm_hist, info);
@@ -588,8 +590,11 @@ int iemic_newton_step_many(iemic_ctx *const *ctx, int count, const double *const
                            const iemic_solver_params *p, iemic_solve_info *infos, int concurrent)
 {
     if (!ctx || !d_x || !d_dx || count < 1) return set_error(IEMIC_EINVAL, "bad model list");
-    for (int q = 0; q < count; q++)
+    for (int q = 0; q < count; q++) {
         if (!ctx[q] || !d_x[q] || !d_dx[q]) return set_error(IEMIC_EINVAL, "null model or vector");
+        for (int r = 0; r < q; r++)
+            if (ctx[r] == ctx[q]) return set_error(IEMIC_EINVAL, "a context appears twice in the model list");
+    }
     std::vector<int> rcs(count, IEMIC_OK);
     std::vector<std::string> errs(count);
     std::vector<iemic_solve_info> inf(count);

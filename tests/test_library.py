@@ -25,3 +25,15 @@ def test_binding_signatures_cover_the_header():
     lib = iemic_amd.load()
     for n in declared_functions():
         assert getattr(lib, n).argtypes is not None, n
+
+
+def test_argument_checks_without_a_gpu():
+    """Entry points reject bad arguments before touching the device."""
+    import ctypes as C
+    import iemic_amd
+    lib = iemic_amd.load()
+    dummy = (C.c_void_p * 2)(C.c_void_p(1), C.c_void_p(1))
+    ptrs = (C.c_void_p * 2)(C.c_void_p(8), C.c_void_p(16))
+    rc = lib.iemic_newton_step_many(dummy, 2, ptrs, ptrs, None, None, 1)
+    assert rc != 0 and b"twice" in lib.iemic_last_error()
+    assert lib.iemic_newton_step_many(None, 0, None, None, None, None, 0) != 0
