@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -1096,12 +1097,57 @@ int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y)
     return IEMIC_OK;
 }
 
+static void spmm_launch(iemic_ctx *c, int nc, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy)
+{
+    const dim3 g(nblk(c->sc.ncell, 256), (nrhs + nc - 1) / nc);
+    if (nc == 1) hipLaunchKernelGGL(k_spmm<1>, g, dim3(256), 0, c->stream, c->sc, c->d_J, nrhs, d_X, ldx, d_Y, ldy);
+    else if (nc == 2) hipLaunchKernelGGL(k_spmm<2>, g, dim3(256), 0, c->stream, c->sc, c->d_J, nrhs, d_X, ldx, d_Y, ldy);
+    else hipLaunchKernelGGL(k_spmm<4>, g, dim3(256), 0, c->stream, c->sc, c->d_J, nrhs, d_X, ldx, d_Y, ldy);
+}
+
+// Columns per thread of the SpMM (1, 2 or 4: more columns read the slots
+// fewer times but hold 6 NC accumulators over the unrolled stencil), timed
+// on the real operands at the first product with a given column count (Y
+// is simply rewritten) unless IEMIC_NO_TUNE is set.
+static int spmm_tuned_nc(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy)
+{
+    auto it = c->spmm_nc.find(nrhs);
+    if (it != c->spmm_nc.end()) return it->second;
+    if (nrhs == 1 || std::getenv("IEMIC_NO_TUNE") || d_X == d_Y) return c->spmm_nc[nrhs] = 1;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return 1;
+    }
+    int best = 1;
+    float tbest = 1e30f;
+    for (int nc : {1, 2, 4}) {
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            (void)hipEventRecord(e0, c->stream);
+            spmm_launch(c, nc, nrhs, d_X, ldx, d_Y, ldy);
+            (void)hipEventRecord(e1, c->stream);
+            float ms = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+            tmin = std::min(tmin, ms);
+        }
+        if (tmin < tbest) {
+            tbest = tmin;
+            best = nc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGetLastError();
+    return c->spmm_nc[nrhs] = best;
+}
+
 int launch_spmm(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy)
 {
-    constexpr int NC = 4;
+    const int nc = spmm_tuned_nc(c, nrhs, d_X, ldx, d_Y, ldy);
     timed_begin(c, 2);
-    hipLaunchKernelGGL(k_spmm<NC>, dim3(nblk(c->sc.ncell, 256), (nrhs + NC - 1) / NC), dim3(256), 0, c->stream, c->sc,
-                       c->d_J, nrhs, d_X, ldx, d_Y, ldy);
+    spmm_launch(c, nc, nrhs, d_X, ldx, d_Y, ldy);
     timed_end(c, 2);
     HIPCHK(hipGetLastError());
     if (c->sc.rowintcon >= 0)

@@ -320,6 +320,15 @@ int iemic_apply_tuning(iemic_ctx *c, int nrhs, int *blocks)
     return IEMIC_OK;
 }
 
+int iemic_spmm_tuning(iemic_ctx *c, int nrhs, int *columns)
+{
+    CHECK_CTX(c);
+    if (!columns) return set_error(IEMIC_EINVAL, "null pointer");
+    auto it = c->spmm_nc.find(nrhs);
+    *columns = it == c->spmm_nc.end() ? 0 : it->second;
+    return IEMIC_OK;
+}
+
 int iemic_set_precond_reuse(iemic_ctx *c, int max_age)
 {
     CHECK_CTX(c);

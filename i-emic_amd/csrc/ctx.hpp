@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,7 @@ struct iemic_ctx {
     unsigned jac_version = 0;        // bumped by every Jacobian assembly
     int prec_kind = IEMIC_PREC_ROW_LU;
     unsigned prec_max_age = 0;       // factors may be this many Jacobians old
+    std::map<int, int> spmm_nc;      // tuned columns per thread of the SpMM, per column count
     iemic::Precond *prec = nullptr;
     iemic::RowLU *rowlu = nullptr;
     iemic::Krylov *kry = nullptr;

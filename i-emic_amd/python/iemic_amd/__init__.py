@@ -123,6 +123,7 @@ def load():
         "iemic_continuation_defaults": ([vp], None),
         "iemic_set_precond_reuse": ([vp, C.c_int], C.c_int),
         "iemic_apply_tuning": ([vp, C.c_int, ip], C.c_int),
+        "iemic_spmm_tuning": ([vp, C.c_int, ip], C.c_int),
         "iemic_read_mask": ([C.c_char_p, C.c_int, C.c_int, C.c_int, ip], C.c_int),
         "iemic_newton_step_many": ([vp, C.c_int, vp, vp, vp, vp, C.c_int], C.c_int),
         "iemic_continuation_dev": ([vp, vp, vp, vp, dp, dp, vp], C.c_int),
@@ -363,6 +364,12 @@ class Ocean:
         """Block target the apply tuned for nrhs right-hand sides (0: none)."""
         b = C.c_int()
         _check(self.lib.iemic_apply_tuning(self.h, int(nrhs), C.byref(b)))
+        return b.value
+
+    def spmmTuning(self, nrhs):
+        """Columns per thread the SpMM chose for nrhs columns (0: none)."""
+        b = C.c_int()
+        _check(self.lib.iemic_spmm_tuning(self.h, int(nrhs), C.byref(b)))
         return b.value
 
     def setPreconditionerReuse(self, max_age):

@@ -151,6 +151,9 @@ int iemic_pressure_projection_dev(iemic_ctx *ctx, double *d_v);
  * products aim at, chosen by timing candidates on the first apply; 0 if
  * not tuned yet (IEMIC_NO_TUNE in the environment keeps the default). */
 int iemic_apply_tuning(iemic_ctx *ctx, int nrhs, int *blocks);
+/* columns per thread the SpMM J X chose for nrhs columns (1, 2 or 4, timed
+ * at the first product; 0 if none yet) */
+int iemic_spmm_tuning(iemic_ctx *ctx, int nrhs, int *columns);
 
 /* reuse of row-LU factors across Jacobians (the reference's Newton-Chord
  * hybrid, "enable Newton Chord hybrid solve", Continuation.H:1360): the
