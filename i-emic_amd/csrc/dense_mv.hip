@@ -228,6 +228,76 @@ hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, si
     return hipSuccess;
 }
 
+// Y = alpha A^T X + beta Y with A column-major (m x k): output row c is the
+// dot product of column c of A (contiguous) with each column of X.  One
+// wavefront per output row walks its column 64 rows at a time; the four
+// wavefronts of a block share the chunk of X staged in LDS.
+constexpr int MT_CHUNK = 1024;   // rows of X staged per step
+
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_mmt(int m, int k, int q, const T *__restrict__ A, size_t lda,
+                                             const T *__restrict__ X, size_t ldx, T alpha, T beta,
+                                             T *__restrict__ Y, size_t ldy)
+{
+    __shared__ T xs[NC][MT_CHUNK];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int c = blockIdx.x * 4 + wave;
+    T acc[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) acc[j] = T(0);
+    for (int r0 = 0; r0 < m; r0 += MT_CHUNK) {
+        const int nr = min(MT_CHUNK, m - r0);
+        for (int e = threadIdx.x; e < NC * MT_CHUNK; e += 256) {
+            const int j = e / MT_CHUNK, rr = e % MT_CHUNK;
+            xs[j][rr] = (rr < nr && j < q) ? X[(size_t)j * ldx + r0 + rr] : T(0);
+        }
+        __syncthreads();
+        if (c < k) {
+            const T *a = A + (size_t)c * lda + r0;
+            for (int rr = lane; rr < nr; rr += 64) {
+                const T av = a[rr];
+#pragma unroll
+                for (int j = 0; j < NC; j++) acc[j] += av * xs[j][rr];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        T v = acc[j];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[j] = v;
+    }
+    if (c < k && lane < NC && lane < q) {
+        T v = acc[0];
+#pragma unroll
+        for (int j = 1; j < NC; j++)
+            if (lane == j) v = acc[j];
+        T *y = Y + (size_t)lane * ldy + c;
+        *y = beta == T(0) ? alpha * v : alpha * v + beta * *y;
+    }
+}
+
+// Y = alpha A^T X + beta Y, q <= 8 columns; A column-major m x k (lda)
+template <typename T>
+hipError_t dense_mm_t(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
+                      T beta, T *Y, size_t ldy)
+{
+    if (k <= 0 || q <= 0) return hipSuccess;
+    if (q > 8) return hipErrorInvalidValue;
+    const dim3 g((k + 3) / 4), b(256);
+    if (q == 1) hipLaunchKernelGGL((k_mmt<T, 1>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    else if (q <= 2) hipLaunchKernelGGL((k_mmt<T, 2>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    else if (q <= 4) hipLaunchKernelGGL((k_mmt<T, 4>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    else hipLaunchKernelGGL((k_mmt<T, 8>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    return hipGetLastError();
+}
+
+template hipError_t dense_mm_t<double>(hipStream_t, int, int, int, double, const double *, size_t, const double *,
+                                       size_t, double, double *, size_t);
+template hipError_t dense_mm_t<float>(hipStream_t, int, int, int, float, const float *, size_t, const float *,
+                                      size_t, float, float *, size_t);
+
 // elements of workspace dense_mm needs for products with m rows
 size_t dense_mm_work(int m) { return (size_t)32 * 32 * std::max(m, 1); }
 

@@ -51,6 +51,9 @@ template <typename T>
 hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
                     T beta, T *Y, size_t ldy, T *work, size_t work_elems, int blocks);
 size_t dense_mm_work(int m);
+template <typename T>
+hipError_t dense_mm_t(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
+                      T beta, T *Y, size_t ldy);
 std::vector<uint8_t> pressure_pins(const iemic_ctx *c, const std::vector<int> &k0, const std::vector<int> &rank);
 
 template <typename T>
@@ -676,7 +679,11 @@ static int rowlu_sweep_t(iemic_ctx *c, const RowFactors<T> &F, const double *in,
     for (int j = 0; j + 1 < p->m; j++) {
         const int nr = p->nj[j], nn = p->nj[j + 1];
         if (nr == 0 || nn == 0) continue;
-        BLASCHK(gemv_t(p->blas, nr, nn, (T)-1, F.blk + p->xoff[j + 1], v + p->ubase[j], (T)1, v + p->ubase[j + 1]));
+        if (p->blas_apply)
+            BLASCHK(gemv_t(p->blas, nr, nn, (T)-1, F.blk + p->xoff[j + 1], v + p->ubase[j], (T)1, v + p->ubase[j + 1]));
+        else
+            HIPCHK(dense_mm_t<T>(c->stream, nr, nn, 1, (T)-1, F.blk + p->xoff[j + 1], nr, v + p->ubase[j], nr, (T)1,
+                                 v + p->ubase[j + 1], nn));
     }
     for (int j = p->m - 1; j >= 0; j--) {
         const int nr = p->nj[j];
@@ -687,7 +694,8 @@ static int rowlu_sweep_t(iemic_ctx *c, const RowFactors<T> &F, const double *in,
                                p->d_cptr + p->ubase[j], p->d_cent, p->d_lval, vj, v + p->ubase[j + 1], t);
         else
             HIPCHK(hipMemcpyAsync(t, vj, (size_t)nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        BLASCHK(gemv_t(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
+        if (p->blas_apply) BLASCHK(gemv_t(p->blas, nr, nr, (T)1, F.blk + p->doff[j], t, (T)0, vj));
+        else HIPCHK(dense_mm_t<T>(c->stream, nr, nr, 1, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, nr));
     }
     hipLaunchKernelGGL(k_row_scatter<T>, dim3(nblk(p->nocean, 256)), dim3(256), 0, c->stream, p->nocean, p->d_perm,
                        p->d_rs, v, out);
