@@ -37,3 +37,19 @@ def test_argument_checks_without_a_gpu():
     rc = lib.iemic_newton_step_many(dummy, 2, ptrs, ptrs, None, None, 1)
     assert rc != 0 and b"twice" in lib.iemic_last_error()
     assert lib.iemic_newton_step_many(None, 0, None, None, None, None, 0) != 0
+
+
+def test_default_parameters_follow_the_reference():
+    """iemic_continuation_defaults follows Continuation::getDefaultInitParameters
+    (Continuation.H:1330-1385): initial step 1e-2, minimum 1e-8, maximum 1e3,
+    epsilon increment 1e-5, maximum Newton iterations 7, optimal 3.5, Newton
+    tolerance 1e-4, state tangent scaling 1, predictor bound 1e3, parameter
+    'Combined Forcing' (COMB, index 19 of par.F90).  The solver defaults
+    solve to Ocean::solve's 1e-8."""
+    import iemic_amd
+    cp = iemic_amd.Ocean.continuation_params()
+    assert (cp.par, cp.ds, cp.ds_min, cp.ds_max) == (19, 1e-2, 1e-8, 1e3)
+    assert (cp.eps, cp.newton_maxit, cp.opt_newton, cp.newton_tol) == (1e-5, 7, 3.5, 1e-4)
+    assert (cp.tan_scaling, cp.predictor_bound) == (1.0, 1e3)
+    sp = iemic_amd.Ocean.solver_params()
+    assert sp.tol == 1e-8 and sp.precision == 0
