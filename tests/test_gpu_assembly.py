@@ -71,3 +71,27 @@ def test_gpu_parameter_changes_follow_setparcs():
     _check_F(F, Fo, case, orc.coef(), orc.rowintcon())
     oc.computeJacobian(case.x)
     np.testing.assert_array_equal(oc.getJacobian()[2], orc.jacobian()[2])
+
+
+@pytest.mark.parametrize("grid", [(96, 38, 12), (180, 80, 24)])
+def test_gpu_matches_oracle_on_the_benchmark_basins(grid):
+    """The seeded global basins of the benchmark (mask seed 7, noise seed 11,
+    iemic_amd.basins) at full size: F, the whole Jacobian on the reference
+    graph (35.8M entries at 180x80x24) and diag B equal the oracle's bit for
+    bit (F's integral row, a device reduction, to 1e-12 of its terms)."""
+    from iemic_amd import basins
+    from thcm_cases import Case, Config
+    n, m, l = grid
+    cfg_d, lm, frc, x0, pars = basins.make_global(n, m, l)
+    case = Case(f"global{n}", Config(**{f: cfg_d[f] for f, _ in Config._fields_}), lm, frc, pars, x0)
+    print(f"basin {n}x{m}x{l}", flush=True)
+    oc, orc = gpu_ocean(case), OracleModel(case)
+    _check_F(oc.computeRHS(case.x), orc.rhs(), case, orc.coef(), orc.rowintcon())
+    oc.computeJacobian(case.x)
+    rp, ci, val, dB = oc.getJacobian()
+    rp2, ci2, val2, dB2, _ = orc.jacobian()
+    np.testing.assert_array_equal(rp, rp2)
+    np.testing.assert_array_equal(ci, ci2)
+    bad = np.nonzero(val != val2)[0]
+    assert bad.size == 0, (bad[:10], val[bad[:5]], val2[bad[:5]])
+    np.testing.assert_array_equal(dB, dB2)
