@@ -1,18 +1,33 @@
-// Dense LU with partial pivoting for the row-block preconditioner
+// Dense LU with tournament pivoting for the row-block preconditioner
 // (rowlu.hip), same contract as LAPACK getrf: A = P L U in place, L unit
 // lower, ipiv 1-based (row i was interchanged with row ipiv[i]), info = first
-// zero pivot (1-based) or 0.
+// zero pivot (1-based) or 0, so LAPACK-style solvers (rocSOLVER getrs) apply
+// the factors as is; dense_inverse forms A^{-1} from them directly.
 //
-// Right-looking, blocked twice.  Outer blocks of NB columns update the
-// trailing matrix with rocBLAS TRSM/GEMM (MFMA).  Inside an outer block,
-// sub-panels of IB columns are factored by one workgroup that keeps the whole
-// sub-panel in registers (1024 threads x RPT rows x IB columns): the pivot
-// search is a wavefront/LDS reduction, the pivot row goes through LDS, and
-// no kernel launch separates two columns.  The rest of the outer block is
-// brought up to date after each sub-panel by two kernels (row interchanges,
-// then the IB x w triangular solve and the rank-IB update).  This replaces
-// the column-by-column panel kernels whose launch latency dominated the
-// factorisation (three launches per column).
+// Why not rocSOLVER getrf: its panel factorisation launches three kernels
+// per column (pivot search, scale/update, singularity check) of 2-4 us each,
+// and at the benchmark sizes those latencies are more than half of the
+// preconditioner build.  Here the pivots of a whole sub-panel of CA_W
+// columns are chosen at once by tournament pivoting (communication-avoiding
+// LU, Grigori, Demmel and Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)):
+//   1. every group of CA_G rows runs Gaussian elimination with partial
+//      pivoting on its rows of the sub-panel, in registers, and nominates the
+//      CA_W rows it pivots on;
+//   2. the nominees are merged CA_FAN sets at a time by the same elimination
+//      on their (original) rows until one set is left;
+//   3. those rows are moved to the top, in the order chosen, and the
+//      sub-panel is factored without further pivoting: U11 is what the last
+//      elimination computed, L11 follows from the chosen rows by one
+//      triangular solve against U11, and L21 = A21 U11^{-1}.
+// The pivot rows of a panel then span its column space whenever the panel
+// has full column rank, and the element growth stays close to partial
+// pivoting in practice; only the pivot order differs from LAPACK's.
+//
+// Sub-panels are grouped into outer blocks of CA_NB columns: inside a block
+// the sub-panel's rank-CA_W update reaches the block's columns only; after
+// the block, rocBLAS TRSM and GEMM (matrix cores) update the trailing
+// matrix with depth CA_NB.  Per sub-panel there are 5-6 launches instead of
+// three per column.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -25,215 +40,366 @@
 
 namespace iemic {
 
-constexpr int LU_THREADS = 1024;
-constexpr int LU_IB = 16;       // widest sub-panel
-constexpr int LU_NB = 128;      // outer block width
+constexpr int CA_W = 32;               // sub-panel width
+constexpr int CA_G = 512;              // rows per selection group (one row per thread)
+constexpr int CA_FAN = CA_G / CA_W;    // candidate sets merged per group above the first level
+constexpr int CA_NB = 256;             // outer block width (TRSM/GEMM depth)
+constexpr int CA_MOVES = 2 * CA_W;     // rows a sub-panel's interchanges can touch
+constexpr int CA_STRIP = 16;           // columns per workgroup in the U12 solve and block update
 
-// Factor the m x nc sub-panel at P (leading dimension lda), nc <= IB,
-// with partial pivoting.  Writes the factored rows back in pivoted order and
-// ipiv[0..nc) (1-based, relative to the sub-panel's first row, plus rowoff).
-// RPT x IB values per thread stay in registers: a 1024-thread workgroup has
-// 128 registers per lane, so IB shrinks as the panel gets taller.
-template <typename T, int RPT, int IB>
-__global__ void __launch_bounds__(LU_THREADS) k_lu_panel(T *__restrict__ Pm, int lda, int m, int nc, int rowoff,
-                                                         int *__restrict__ ipiv, int *__restrict__ info, int coloff)
+// Workspace: two candidate lists, the CA_W x CA_W factors of the chosen
+// rows, the move list (count, destinations, sources).
+static size_t cand_elems(int n) { return (size_t)((n + CA_G - 1) / CA_G) * CA_W; }
+
+size_t dense_getrf_work_bytes(int n)
 {
-    __shared__ T prow[IB];
-    __shared__ T wv[LU_THREADS / 64];
-    __shared__ int wi[LU_THREADS / 64];
-    __shared__ int s_piv;
-    __shared__ int prows[IB];
-    __shared__ int trk_row[2 * IB], trk_pos[2 * IB];
-    __shared__ int ntrk;
+    return 2 * cand_elems(n) * sizeof(int) + (size_t)CA_W * CA_W * sizeof(double) + (1 + 2 * CA_MOVES) * sizeof(int) +
+           256;
+}
+
+// wave64 maximum of unsigned 32-bit values, returned to every lane: DPP row
+// shifts by 1, 2, 4, 8 (lane 15 of each row of 16 then holds the row's
+// maximum), row broadcasts of lanes 15 and 31, and lane 63 read out.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned x)
+{
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x111, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x112, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xa, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xc, 0xf, false));
+    return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// The value v of lane `src` (uniform) in every lane, through scalar registers.
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_value(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// Order-preserving key of |v| over the lanes of a wave: the bits of |v| (a
+// non-negative double orders like its bit pattern) with the top bit set on
+// live lanes, 0 on the others.  Returns the first lane holding the largest
+// key (-1 when no lane is live) and the key itself.
+template <typename T>
+__device__ __forceinline__ int wave_argmax_abs(T v, bool live, unsigned long long *key)
+{
+    unsigned hi, lo;
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs((double)v));
+        hi = (unsigned)(b >> 32);
+        lo = (unsigned)b;
+    } else {
+        hi = (unsigned)__float_as_uint(fabsf((float)v));
+        lo = 0u;
+    }
+    hi = live ? (hi | 0x80000000u) : 0u;
+    const unsigned mh = wave_max_u32(hi);
+    const unsigned ml = sizeof(T) == 8 ? wave_max_u32(hi == mh ? lo : 0u) : 0u;
+    const unsigned long long bal = __ballot(hi == mh && lo == ml);
+    *key = ((unsigned long long)mh << 32) | ml;
+    return mh ? (int)__ffsll(bal) - 1 : -1;
+}
+
+// Gaussian elimination with partial pivoting on the rows of one group,
+// restricted to the sub-panel columns [k, k + w): nominates up to w rows, in
+// pivot order (largest magnitude; on ties the group's first row).  The group's
+// rows are rows_in[g CA_G + t] (absolute row indices, -1 = none), or
+// k + g CA_G + t when rows_in is null.  At the final level (one group) the
+// workgroup also writes the factors of the chosen rows (lu11, column-major,
+// leading dimension CA_W), the LAPACK interchanges ipiv[k .. k + w), the move
+// list that performs them, and info at the first zero pivot.
+template <typename T>
+__global__ void __launch_bounds__(CA_G)
+    k_calu_select(const T *__restrict__ A, int lda, int k, int w, const int *__restrict__ rows_in, int cnt,
+                  int *__restrict__ rows_out, int final_level, T *__restrict__ lu11, int *__restrict__ ipiv,
+                  int *__restrict__ moves, int *__restrict__ info)
+{
+    // per wave and column parity: the wave's best row (its key, thread, and
+    // values), so that one barrier per column suffices
+    __shared__ T wrow[2][CA_G / 64][CA_W];
+    __shared__ unsigned long long wk[2][CA_G / 64];
+    __shared__ int wt[2][CA_G / 64];
+    __shared__ T s_u[CA_W][CA_W + 1];   // rows of U11 as they are chosen
+    __shared__ int s_sel[CA_W];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    T a[RPT][IB];
-    bool used[RPT];
+    const int idx = blockIdx.x * CA_G + t;
+    const int myrow = idx < cnt ? (rows_in ? rows_in[idx] : k + idx) : -1;
+    bool live = myrow >= 0;
+    // The row stays in registers.  The elimination updates all 32 columns
+    // unconditionally (columns <= c are never read again, so what they hold
+    // does not matter), and the multipliers are not kept: the final level
+    // recomputes L11 from the chosen rows.  Columns are visited in blocks of
+    // 8, unrolled inside a block, so column c is one of four registers.  (A
+    // fully unrolled 32-column loop is ~20k instructions of straight-line
+    // code, beyond the instruction cache.)
+    T a[CA_W];
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int r = t + LU_THREADS * q;
-        used[q] = false;
+    for (int c = 0; c < CA_W; c++) a[c] = (live && c < w) ? A[myrow + (size_t)(k + c) * lda] : (T)0;
+    if (t < CA_W) s_sel[t] = -1;
+    __syncthreads();
+    bool done = false;   // uniform: no rows left in this group
+#pragma unroll 1
+    for (int cb = 0; cb < w && !done; cb += 8) {
 #pragma unroll
-        for (int c = 0; c < IB; c++) a[q][c] = (r < m && c < nc) ? Pm[r + (size_t)c * lda] : (T)0;
-    }
-    for (int c = 0; c < nc; c++) {
-        // pivot search: largest magnitude, smallest row on ties (LAPACK iamax)
-        T best = (T)-1;
-        int bi = INT_MAX;
+        for (int j = 0; j < 8; j++) {
+            const int c = cb + j;
+            if (c >= w || done) break;
+            const T ac = cb == 0 ? a[j] : cb == 8 ? a[8 + j] : cb == 16 ? a[16 + j] : a[24 + j];
+            const int par = c & 1;
+            unsigned long long key;
+            const int fl = wave_argmax_abs(ac, live, &key);
+            if (lane == fl) {
 #pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            const int r = t + LU_THREADS * q;
-            T v = (T)-1;
-#pragma unroll
-            for (int cc = 0; cc < IB; cc++)
-                if (cc == c) v = fabs(a[q][cc]);
-            if (r < m && !used[q] && (v > best || (v == best && r < bi))) {
-                best = v;
-                bi = r;
+                for (int cc = 0; cc < CA_W; cc++) wrow[par][wave][cc] = a[cc];
             }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) {
-                best = ov;
-                bi = oi;
+            if (lane == 0) {
+                wk[par][wave] = key;
+                wt[par][wave] = wave * 64 + (fl < 0 ? 0 : fl);
             }
-        }
-        if (lane == 0) {
-            wv[wave] = best;
-            wi[wave] = bi;
-        }
-        __syncthreads();
-        if (t == 0) {
-            T b = wv[0];
-            int i = wi[0];
-            for (int w = 1; w < LU_THREADS / 64; w++)
-                if (wv[w] > b || (wv[w] == b && wi[w] < i)) {
-                    b = wv[w];
-                    i = wi[w];
+            __syncthreads();
+            unsigned long long bk = wk[par][0];
+            int bw = 0;
+#pragma unroll
+            for (int q = 1; q < CA_G / 64; q++)
+                if (wk[par][q] > bk) {
+                    bk = wk[par][q];
+                    bw = q;
                 }
-            s_piv = i;
-            prows[c] = i;
-        }
-        __syncthreads();
-        const int pr = s_piv;
-        if (pr % LU_THREADS == t) {
+            if (bk == 0) {   // uniform
+                done = true;
+                break;
+            }
+            const int pt = wt[par][bw];
+            // the pivot row to every lane: lane i reads element i once, the
+            // values are then broadcast from lanes into scalar registers (a
+            // broadcast LDS read of the whole row by every lane would move
+            // 64 copies of it through the LDS return path)
+            const T myv = wrow[par][bw][lane & (CA_W - 1)];
+            const T piv = lane_value(myv, c);
+            if (t == pt) {
+                if (final_level) {
 #pragma unroll
-            for (int q = 0; q < RPT; q++)
-                if (q == pr / LU_THREADS) {
-#pragma unroll
-                    for (int cc = 0; cc < IB; cc++) prow[cc] = a[q][cc];
-                    used[q] = true;
+                    for (int cc = 0; cc < CA_W; cc++) s_u[c][cc] = cc >= c ? a[cc] : (T)0;
                 }
-        }
-        __syncthreads();
-        const T piv = prow[c];
-        if (piv == (T)0) {
-            if (t == 0 && *info == 0) *info = coloff + c + 1;
-            continue;
-        }
-        const T rinv = (T)1 / piv;
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            const int r = t + LU_THREADS * q;
-            if (r < m && !used[q]) {
-                T l = 0;
-#pragma unroll
-                for (int cc = 0; cc < IB; cc++)
-                    if (cc == c) {
-                        l = a[q][cc] * rinv;
-                        a[q][cc] = l;
-                    }
-#pragma unroll
-                for (int cc = 0; cc < IB; cc++)
-                    if (cc > c) a[q][cc] -= l * prow[cc];
+                s_sel[c] = myrow;
+                live = false;
             }
+            if (piv == (T)0 && final_level && t == 0 && *info == 0) *info = k + c + 1;
+            // branch-free: rows that are no longer live (and all rows at a
+            // zero pivot) take a zero multiplier and keep their values
+            T pr[CA_W];
+#pragma unroll
+            for (int cc = 0; cc < CA_W; cc++) pr[cc] = lane_value(myv, cc);
+            const T l = (live && piv != (T)0) ? ac / piv : (T)0;
+#pragma unroll
+            for (int cc = 0; cc < CA_W; cc++) a[cc] = fma(-l, pr[cc], a[cc]);
         }
-    }
-    // LAPACK interchange sequence: at step c the rows at positions c and
-    // p_c swap, p_c = current position of pivot row prows[c]
-    if (t == 0) {
-        int n = 0;
-        for (int c = 0; c < nc; c++) {
-            const int row = prows[c];
-            int p = row;
-            for (int e = 0; e < n; e++)
-                if (trk_row[e] == row) p = trk_pos[e];
-            ipiv[c] = rowoff + p + 1;
-            int rc = c;   // row currently at position c
-            for (int e = 0; e < n; e++)
-                if (trk_pos[e] == c) rc = trk_row[e];
-            // row -> position c, rc -> position p
-            bool fr = false, fc = false;
-            for (int e = 0; e < n; e++) {
-                if (trk_row[e] == row) { trk_pos[e] = c; fr = true; }
-                else if (trk_row[e] == rc) { trk_pos[e] = p; fc = true; }
-            }
-            if (!fr) { trk_row[n] = row; trk_pos[n] = c; n++; }
-            if (!fc && rc != row) { trk_row[n] = rc; trk_pos[n] = p; n++; }
-        }
-        ntrk = n;
     }
     __syncthreads();
-    const int nt = ntrk;
+    if (final_level && t < w) {
+        // row t of L11: x U11 = (chosen row t as it is in A), x_p for p < t
+        const int r = s_sel[t];
+        T x[CA_W];
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int r = t + LU_THREADS * q;
-        if (r >= m) continue;
-        int fp = r;
-        for (int e = 0; e < nt; e++)
-            if (trk_row[e] == r) fp = trk_pos[e];
+        for (int c = 0; c < CA_W; c++) x[c] = (r >= 0 && c < w) ? A[r + (size_t)(k + c) * lda] : (T)0;
 #pragma unroll
-        for (int c = 0; c < IB; c++)
-            if (c < nc) Pm[fp + (size_t)c * lda] = a[q][c];
+        for (int p = 0; p < CA_W; p++) {
+            if (p < t) {
+                x[p] = x[p] / s_u[p][p];
+#pragma unroll
+                for (int q = p + 1; q < CA_W; q++) x[q] = fma(-x[p], s_u[p][q], x[q]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < CA_W; p++)
+            if (p < w) lu11[t + p * CA_W] = p < t ? x[p] : s_u[t][p];
     }
-}
-
-// Row interchanges ipiv[k1..k2) (1-based global rows) on columns [c0, c1),
-// one thread per column.
-template <typename T>
-__global__ void k_lu_laswp(T *__restrict__ A, int lda, int c0, int c1, int k1, int k2, const int *__restrict__ ipiv)
-{
-    const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= c1) return;
-    T *a = A + (size_t)col * lda;
-    for (int i = k1; i < k2; i++) {
-        const int p = ipiv[i] - 1;
-        if (p != i) {
-            const T tmp = a[i];
-            a[i] = a[p];
-            a[p] = tmp;
+    __syncthreads();
+    if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
+    if (!final_level || wave != 0) return;
+    // LAPACK interchange sequence and the net row moves.  Lane e holds entry
+    // e of the map "position key[e] now holds original row val[e]" over the
+    // positions touched so far (at most 2 w <= 64).
+    int key = -1, val = -1, size = 0;
+    for (int c = 0; c < w; c++) {
+        const int s = s_sel[c];
+        const unsigned long long bv = __ballot(lane < size && val == s);
+        const int p = bv ? __shfl(key, __ffsll((unsigned long long)bv) - 1, 64) : s;
+        const unsigned long long bk = __ballot(lane < size && key == k + c);
+        const int cur = bk ? __shfl(val, __ffsll((unsigned long long)bk) - 1, 64) : k + c;
+        if (lane == 0) ipiv[k + c] = p + 1;
+        // position k + c receives s, position p receives cur
+        if (bk) {
+            if (lane == __ffsll((unsigned long long)bk) - 1) val = s;
+        } else {
+            if (lane == size) {
+                key = k + c;
+                val = s;
+            }
+            size++;
+        }
+        if (p != k + c) {
+            const unsigned long long bp = __ballot(lane < size && key == p);
+            if (bp) {
+                if (lane == __ffsll((unsigned long long)bp) - 1) val = cur;
+            } else {
+                if (lane == size) {
+                    key = p;
+                    val = cur;
+                }
+                size++;
+            }
         }
     }
+    const bool mv = lane < size && key != val;
+    const unsigned long long bm = __ballot(mv);
+    if (mv) {
+        const int slot = __popcll(bm & ((1ull << lane) - 1));
+        moves[1 + slot] = key;              // destination
+        moves[1 + CA_MOVES + slot] = val;   // source
+    }
+    if (lane == 0) moves[0] = __popcll(bm);
 }
 
-// After a sub-panel at rows/columns [k, k+ib): for the columns [c0, c1) of
-// the outer block, U12 = L11^{-1} A12 on rows [k, k+ib), then
-// A22 -= L21 U12 on rows [k+ib, n).  Grid: (column strips of 16, row chunks);
-// every workgroup solves its strip's small triangular system itself, the
-// row-chunk-0 workgroups write U12.
+// The row moves of one sub-panel on columns [0, ncols) and, as one more
+// column, on the row permutation perm (when given): all sources of a column
+// are read before any destination is written (one column per 64 threads,
+// the barrier orders them).
 template <typename T>
-__global__ void __launch_bounds__(256) k_lu_update(T *__restrict__ A, int lda, int n, int k, int ib, int c0, int c1,
-                                                   int rows_per_chunk)
+__global__ void __launch_bounds__(256)
+    k_calu_swap(T *__restrict__ A, int lda, int ncols, const int *__restrict__ moves, int *__restrict__ perm)
 {
-    __shared__ T L11[LU_IB][LU_IB];
-    __shared__ T U[LU_IB][16];
-    const int cs = c0 + blockIdx.x * 16;
-    const int w = min(16, c1 - cs);
+    const int e = threadIdx.x & 63;
+    const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int cnt = moves[0];
+    const bool act = col < ncols && e < cnt;
+    const bool actp = perm && col == ncols && e < cnt;
+    T v = (T)0;
+    int pv = 0;
+    if (act) v = A[moves[1 + CA_MOVES + e] + (size_t)col * lda];
+    if (actp) pv = perm[moves[1 + CA_MOVES + e]];
+    __syncthreads();
+    if (act) A[moves[1 + e] + (size_t)col * lda] = v;
+    if (actp) perm[moves[1 + e]] = pv;
+}
+
+__global__ void k_lu_iota(int *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// W = P^T (the identity with its rows permuted as LAPACK's interchanges
+// permute them): W[i, perm[i]] = 1, zero elsewhere.  Grid-stride over the
+// n x n elements.
+template <typename T>
+__global__ void k_lu_perm_identity(T *__restrict__ W, int n, const int *__restrict__ perm)
+{
+    const size_t tot = (size_t)n * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % n), j = (int)(e / n);
+        W[e] = perm[i] == j ? (T)1 : (T)0;
+    }
+}
+
+// After the interchanges: L21 = A21 U11^{-1} on rows [k + w, n) of the
+// sub-panel (workgroups [0, nrowblk)), the factors of the chosen rows into
+// the top w x w block (workgroup 0), and U12 = L11^{-1} A12 on the rows
+// [k, k + w) of the block columns [k + w, c1) (workgroups nrowblk + strip).
+template <typename T>
+__global__ void __launch_bounds__(256) k_calu_l21(T *__restrict__ A, int lda, int n, int k, int w, int c1,
+                                                  const T *__restrict__ lu11, int nrowblk)
+{
+    __shared__ T s_lu[CA_W][CA_W + 1];
+    __shared__ T s_rd[CA_W];
+    __shared__ T s_b[CA_W][CA_STRIP + 1];
     const int t = threadIdx.x;
-    for (int e = t; e < ib * ib; e += blockDim.x) {
-        const int i = e % ib, p = e / ib;
-        L11[i][p] = A[(k + i) + (size_t)(k + p) * lda];
+    for (int e = t; e < w * w; e += blockDim.x) s_lu[e % w][e / w] = lu11[(e % w) + (e / w) * CA_W];
+    if (t < w) s_rd[t] = (T)1 / lu11[t + t * CA_W];
+    const int b = blockIdx.x;
+    if (b < nrowblk) {
+        if (b == 0)
+            for (int e = t; e < w * w; e += blockDim.x)
+                A[(k + e % w) + (size_t)(k + e / w) * lda] = lu11[(e % w) + (e / w) * CA_W];
+        __syncthreads();
+        const int r = k + w + b * blockDim.x + t;
+        if (r >= n) return;
+        // x U11 = a, column by column: x_c = a_c / u_cc, then a_q -= x_c u_cq
+        T a[CA_W];
+#pragma unroll
+        for (int c = 0; c < CA_W; c++) a[c] = c < w ? A[r + (size_t)(k + c) * lda] : (T)0;
+#pragma unroll
+        for (int c = 0; c < CA_W; c++) {
+            if (c < w) {
+                const T x = a[c] * s_rd[c];
+                a[c] = x;
+#pragma unroll
+                for (int q = c + 1; q < CA_W; q++) a[q] -= x * s_lu[c][q];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CA_W; c++)
+            if (c < w) A[r + (size_t)(k + c) * lda] = a[c];
+        return;
     }
-    for (int e = t; e < ib * 16; e += blockDim.x) {
-        const int i = e % ib, cc = e / ib;
-        U[i][cc] = cc < w ? A[(k + i) + (size_t)(cs + cc) * lda] : (T)0;
+    const int cs = k + w + (b - nrowblk) * CA_STRIP;
+    const int nc = min(CA_STRIP, c1 - cs);
+    for (int e = t; e < w * CA_STRIP; e += blockDim.x) {
+        const int i = e % w, cc = e / w;
+        s_b[i][cc] = cc < nc ? A[(k + i) + (size_t)(cs + cc) * lda] : (T)0;
     }
     __syncthreads();
-    if (t < 16) {   // forward substitution with the unit lower L11, one column per thread
-        for (int i = 1; i < ib; i++) {
-            T s = U[i][t];
-            for (int p = 0; p < i; p++) s -= L11[i][p] * U[p][t];
-            U[i][t] = s;
+    if (t < nc) {
+        for (int i = 1; i < w; i++) {
+            T s = s_b[i][t];
+            for (int p = 0; p < i; p++) s -= s_lu[i][p] * s_b[p][t];
+            s_b[i][t] = s;
         }
     }
     __syncthreads();
-    if (blockIdx.y == 0)
-        for (int e = t; e < ib * 16; e += blockDim.x) {
-            const int i = e % ib, cc = e / ib;
-            if (cc < w) A[(k + i) + (size_t)(cs + cc) * lda] = U[i][cc];
-        }
-    const int r0 = k + ib + blockIdx.y * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
+    for (int e = t; e < w * CA_STRIP; e += blockDim.x) {
+        const int i = e % w, cc = e / w;
+        if (cc < nc) A[(k + i) + (size_t)(cs + cc) * lda] = s_b[i][cc];
+    }
+}
+
+// Rank-w update of the block columns [c0, c1) on rows [k + w, n):
+// A22 -= L21 U12, with L21 and U12 already in A.  Grid: (column strips,
+// row chunks of rows_per_chunk).
+template <typename T>
+__global__ void __launch_bounds__(256) k_calu_update(T *__restrict__ A, int lda, int n, int k, int w, int c0, int c1,
+                                                     int rows_per_chunk)
+{
+    __shared__ T U[CA_W][CA_STRIP];
+    const int cs = c0 + blockIdx.x * CA_STRIP;
+    const int nc = min(CA_STRIP, c1 - cs);
+    const int t = threadIdx.x;
+    for (int e = t; e < CA_W * CA_STRIP; e += blockDim.x) {
+        const int i = e % CA_W, cc = e / CA_W;
+        U[i][cc] = (i < w && cc < nc) ? A[(k + i) + (size_t)(cs + cc) * lda] : (T)0;
+    }
+    __syncthreads();
+    const int r0 = k + w + blockIdx.y * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
     for (int r = r0 + t; r < r1; r += blockDim.x) {
-        T l[LU_IB];
+        T acc[CA_STRIP];
 #pragma unroll
-        for (int p = 0; p < LU_IB; p++) l[p] = p < ib ? A[r + (size_t)(k + p) * lda] : (T)0;
-        for (int cc = 0; cc < w; cc++) {
-            T s = A[r + (size_t)(cs + cc) * lda];
+        for (int cc = 0; cc < CA_STRIP; cc++) acc[cc] = cc < nc ? A[r + (size_t)(cs + cc) * lda] : (T)0;
+#pragma unroll 4
+        for (int p = 0; p < w; p++) {
+            const T l = A[r + (size_t)(k + p) * lda];
 #pragma unroll
-            for (int p = 0; p < LU_IB; p++) s -= l[p] * U[p][cc];
-            A[r + (size_t)(cs + cc) * lda] = s;
+            for (int cc = 0; cc < CA_STRIP; cc++) acc[cc] -= l * U[p][cc];
         }
+#pragma unroll
+        for (int cc = 0; cc < CA_STRIP; cc++)
+            if (cc < nc) A[r + (size_t)(cs + cc) * lda] = acc[cc];
     }
 }
 
@@ -262,77 +428,88 @@ static rocblas_status gemm_nn_sub(rocblas_handle h, int m, int n, int k, const f
     return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &mone, A, lda, B, ldb, &one, C, ldc);
 }
 
-template <typename T, int RPT, int IB>
-static void launch_panel(hipStream_t s, T *P, int lda, int m, int nc, int rowoff, int *ipiv, int *info, int coloff)
+// Inside rocblas_start/stop_device_memory_size_query: the rocBLAS calls of
+// dense_getrf at their largest shapes for an n x n block, so that the
+// handle's workspace covers them.
+void dense_getrf_size_query(rocblas_handle h, int n)
 {
-    hipLaunchKernelGGL((k_lu_panel<T, RPT, IB>), dim3(1), dim3(LU_THREADS), 0, s, P, lda, m, nc, rowoff, ipiv, info,
-                       coloff);
-}
-
-// Sub-panel width for an m-row panel, so that the RPT x IB values of a lane
-// stay in its 128 registers (no scratch): FP64 24 values, FP32 48.
-template <typename T>
-static int panel_width(int m)
-{
-    const int rpt = (m + LU_THREADS - 1) / LU_THREADS;
-    if (sizeof(T) == 8) return rpt <= 1 ? 16 : rpt <= 3 ? 8 : rpt <= 6 ? 4 : rpt <= 12 ? 2 : 1;
-    return rpt <= 2 ? 16 : rpt <= 6 ? 8 : rpt <= 12 ? 4 : rpt <= 24 ? 2 : 1;
-}
-
-template <typename T>
-static int panel(hipStream_t s, T *P, int lda, int m, int nc, int rowoff, int *ipiv, int *info, int coloff)
-{
-    const int rpt = (m + LU_THREADS - 1) / LU_THREADS;
-    const int ib = panel_width<T>(m);
-    if (nc > ib) return set_error(IEMIC_EINVAL, "dense LU: sub-panel wider than its register budget");
-#define LU_CASE(R, W) \
-    if (rpt <= R && ib == W) { launch_panel<T, R, W>(s, P, lda, m, nc, rowoff, ipiv, info, coloff); return IEMIC_OK; }
-    if constexpr (sizeof(T) == 8) {
-        LU_CASE(1, 16) LU_CASE(3, 8) LU_CASE(6, 4) LU_CASE(12, 2) LU_CASE(24, 1)
-    } else {
-        LU_CASE(2, 16) LU_CASE(6, 8) LU_CASE(12, 4) LU_CASE(24, 2) LU_CASE(24, 1)
+    double *dd = nullptr;
+    float *df = nullptr;
+    const double one = 1.0;
+    const float onef = 1.0f;
+    if (n > CA_NB) {
+        (void)trsm_llnu(h, CA_NB, n - CA_NB, dd, n, dd, n);
+        (void)gemm_nn_sub(h, n - CA_NB, n - CA_NB, CA_NB, dd, n, dd, n, dd, n);
+        (void)trsm_llnu(h, CA_NB, n - CA_NB, df, n, df, n);
+        (void)gemm_nn_sub(h, n - CA_NB, n - CA_NB, CA_NB, df, n, df, n, df, n);
     }
-#undef LU_CASE
-    return set_error(IEMIC_EUNSUPPORTED, "dense LU: panel taller than 24576 rows");
+    // the n x n solves of dense_inverse
+    (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, n, n,
+                        &one, dd, n, dd, n);
+    (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, n,
+                        n, &one, dd, n, dd, n);
+    (void)rocblas_strsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, n, n,
+                        &onef, df, n, df, n);
+    (void)rocblas_strsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, n,
+                        n, &onef, df, n, df, n);
 }
 
-#define LU_BLASCHK(expr)                                                                            \
+#define LU_BLASCHK(expr)                                                                        \
     do {                                                                                            \
         rocblas_status st_ = (expr);                                                                \
         if (st_ != rocblas_status_success)                                                          \
             return set_error(IEMIC_EHIP, std::string(#expr) + ": status " + std::to_string((int)st_)); \
     } while (0)
 
-// A = P L U for the n x n column-major A (lda = n); ipiv and info on the device.
+// A = P L U for the n x n column-major A (lda = n); ipiv and info on the
+// device; work holds dense_getrf_work_bytes(n) bytes of device memory.
+// When perm is given it receives the row permutation (row i of P^T A is row
+// perm[i] of A).
 template <typename T>
-int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info)
+int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info, void *work, int *perm)
 {
+    if (n <= 0) return IEMIC_OK;
+    if (!work) return set_error(IEMIC_EINVAL, "dense LU: no workspace");
+    const size_t ce = cand_elems(n);
+    int *cand[2] = {reinterpret_cast<int *>(work), reinterpret_cast<int *>(work) + ce};
+    T *lu11 = reinterpret_cast<T *>(reinterpret_cast<char *>(work) + ((2 * ce * sizeof(int) + 255) / 256) * 256);
+    int *moves = reinterpret_cast<int *>(lu11 + CA_W * CA_W);
     HIPCHK(hipMemsetAsync(info, 0, sizeof(int), s));
-    for (int K = 0; K < n; K += LU_NB) {
-        const int nb = std::min(LU_NB, n - K);
-        for (int k = K, ib = 0; k < K + nb; k += ib) {
-            ib = std::min(panel_width<T>(n - k), K + nb - k);
-            int rc = panel<T>(s, A + k + (size_t)k * n, n, n - k, ib, k, ipiv + k, info, k);
-            if (rc) return rc;
-            // interchanges on the rest of the outer block, left and right of the sub-panel
-            if (k > K)
-                hipLaunchKernelGGL(k_lu_laswp<T>, dim3((k - K + 63) / 64), dim3(64), 0, s, A, n, K, k, k, k + ib, ipiv);
-            const int r0 = k + ib, r1 = K + nb;
-            if (r1 > r0) {
-                hipLaunchKernelGGL(k_lu_laswp<T>, dim3((r1 - r0 + 63) / 64), dim3(64), 0, s, A, n, r0, r1, k, k + ib,
-                                   ipiv);
-                const int rows = n - (k + ib);
-                const int chunk = 1024;
-                dim3 grid((r1 - r0 + 15) / 16, std::max(1, (rows + chunk - 1) / chunk));
-                hipLaunchKernelGGL(k_lu_update<T>, grid, dim3(256), 0, s, A, n, n, k, ib, r0, r1, chunk);
+    if (perm) hipLaunchKernelGGL(k_lu_iota, dim3((n + 255) / 256), dim3(256), 0, s, perm, n);
+    for (int K = 0; K < n; K += CA_NB) {
+        const int nb = std::min(CA_NB, n - K);
+        for (int k = K; k < K + nb; k += CA_W) {
+            const int w = std::min(CA_W, K + nb - k);
+            // tournament over rows [k, n)
+            int cnt = n - k, groups = (cnt + CA_G - 1) / CA_G;
+            const int *in = nullptr;
+            int lev = 0;
+            for (;;) {
+                const int fin = groups == 1;
+                int *out = cand[lev & 1];
+                hipLaunchKernelGGL(k_calu_select<T>, dim3(groups), dim3(CA_G), 0, s, A, n, k, w, in, cnt, out, fin,
+                                   lu11, ipiv, moves, info);
+                if (fin) break;
+                in = out;
+                cnt = groups * CA_W;
+                groups = (cnt + CA_G - 1) / CA_G;
+                lev++;
+            }
+            hipLaunchKernelGGL(k_calu_swap<T>, dim3((n + 1 + 3) / 4), dim3(256), 0, s, A, n, n, moves, perm);
+            const int below = n - (k + w);
+            const int nrowblk = std::max(1, (below + 255) / 256);
+            const int c1 = K + nb;
+            const int strips = (c1 - (k + w) + CA_STRIP - 1) / CA_STRIP;
+            hipLaunchKernelGGL(k_calu_l21<T>, dim3(nrowblk + strips), dim3(256), 0, s, A, n, n, k, w, c1, lu11,
+                               nrowblk);
+            if (strips > 0 && below > 0) {
+                const int chunk = 256;
+                dim3 grid(strips, (below + chunk - 1) / chunk);
+                hipLaunchKernelGGL(k_calu_update<T>, grid, dim3(256), 0, s, A, n, n, k, w, k + w, c1, chunk);
             }
         }
-        // outer block: interchanges on the columns outside it, then U12 and the trailing update
-        if (K > 0)
-            hipLaunchKernelGGL(k_lu_laswp<T>, dim3((K + 63) / 64), dim3(64), 0, s, A, n, 0, K, K, K + nb, ipiv);
         const int c0 = K + nb;
         if (c0 < n) {
-            hipLaunchKernelGGL(k_lu_laswp<T>, dim3((n - c0 + 63) / 64), dim3(64), 0, s, A, n, c0, n, K, K + nb, ipiv);
             LU_BLASCHK(trsm_llnu(h, nb, n - c0, A + K + (size_t)K * n, n, A + K + (size_t)c0 * n, n));
             LU_BLASCHK(gemm_nn_sub(h, n - c0, n - c0, nb, A + c0 + (size_t)K * n, n, A + K + (size_t)c0 * n, n,
                                    A + c0 + (size_t)c0 * n, n));
@@ -342,7 +519,37 @@ int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *in
     return IEMIC_OK;
 }
 
-template int dense_getrf<double>(rocblas_handle, hipStream_t, int, double *, int *, int *);
-template int dense_getrf<float>(rocblas_handle, hipStream_t, int, float *, int *, int *);
+static rocblas_status trsm_lunn(rocblas_handle h, int n, const double *U, double *B)
+{
+    const double one = 1.0;
+    return rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                         n, n, &one, U, n, B, n);
+}
+static rocblas_status trsm_lunn(rocblas_handle h, int n, const float *U, float *B)
+{
+    const float one = 1.0f;
+    return rocblas_strsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                         n, n, &one, U, n, B, n);
+}
+
+// W = A^{-1} from the factors of dense_getrf: W = U^{-1} L^{-1} P^T, the
+// permuted identity written directly (no row interchanges on W).
+template <typename T>
+int dense_inverse(rocblas_handle h, hipStream_t s, int n, const T *LU, const int *perm, T *W)
+{
+    if (n <= 0) return IEMIC_OK;
+    const size_t tot = (size_t)n * n;
+    hipLaunchKernelGGL(k_lu_perm_identity<T>, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 8192)), dim3(256), 0,
+                       s, W, n, perm);
+    HIPCHK(hipGetLastError());
+    LU_BLASCHK(trsm_llnu(h, n, n, LU, n, W, n));
+    LU_BLASCHK(trsm_lunn(h, n, LU, W));
+    return IEMIC_OK;
+}
+
+template int dense_getrf<double>(rocblas_handle, hipStream_t, int, double *, int *, int *, void *, int *);
+template int dense_getrf<float>(rocblas_handle, hipStream_t, int, float *, int *, int *, void *, int *);
+template int dense_inverse<double>(rocblas_handle, hipStream_t, int, const double *, const int *, double *);
+template int dense_inverse<float>(rocblas_handle, hipStream_t, int, const float *, const int *, float *);
 
 }  // namespace iemic

@@ -26,9 +26,10 @@
 // The factored matrix is equilibrated first, R M C with power-of-two row and
 // column scales (exact in floating point), and M^{-1} = C (R M C)^{-1} R.
 //
-// Dense LU and inversion are rocSOLVER calls, the matrix-vector products
-// rocBLAS GEMV; the block assembly, the sparse Schur-complement updates and
-// the row gathers are kernels of this file.  The FP32 factors are computed
+// Dense LU is the tournament-pivoted dense_getrf of dense_lu.hip (rocSOLVER
+// getrf + getrs with IEMIC_DENSE_LU=0); the inverse comes from two rocBLAS
+// triangular solves; the block assembly, the sparse Schur-complement updates
+// and the row gathers are kernels of this file.  The FP32 factors are computed
 // in FP32 (from the FP64 slot values) for the mixed-precision solve.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
@@ -46,7 +47,11 @@ namespace iemic {
 
 std::vector<int> column_tops(const iemic_ctx *c);
 template <typename T>
-int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info);
+int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info, void *work, int *perm);
+template <typename T>
+int dense_inverse(rocblas_handle h, hipStream_t s, int n, const T *LU, const int *perm, T *W);
+size_t dense_getrf_work_bytes(int n);
+void dense_getrf_size_query(rocblas_handle h, int n);
 template <typename T>
 hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
                     T beta, T *Y, size_t ldy, T *work, size_t work_elems, int blocks);
@@ -83,6 +88,8 @@ struct RowLU {
     RowFactors<double> f64;
     RowFactors<float> f32;
     int *d_ipiv = nullptr, *d_info = nullptr;
+    void *d_luw = nullptr;               // dense_getrf workspace
+    int *d_rowperm = nullptr;            // its row permutation (nmax)
     void *d_v = nullptr, *d_t = nullptr;  // row-layout work vectors (nocean, nmax) in the widest type
     void *d_vm = nullptr, *d_tm = nullptr; // the same for mcap right-hand sides (multi-vector sweeps)
     int mcap = 0;
@@ -97,7 +104,7 @@ struct RowLU {
     unsigned null_version[2] = {0, 0};
     int null_prec[2] = {-1, -1};
     double *d_unit = nullptr;
-    bool dense_lu = false;                // own dense_getrf instead of rocSOLVER getrf (comparison runs)
+    bool dense_lu = true;                 // tournament-pivoted dense_getrf; false: rocSOLVER getrf + getrs
     bool blas_apply = false;              // rocBLAS gemv/gemm in the sweeps instead of dense_mm (comparison runs)
     void *d_mmw = nullptr;                // dense_mm workspace, dense_mm_work(nmax) doubles
     std::map<int, int> mm_blocks;         // tuned dense_mm block target per number of right-hand sides
@@ -364,7 +371,7 @@ void rowlu_free(iemic_ctx *c)
     void *bufs[] = {p->d_loc, p->d_perm, p->d_lptr, p->d_lent, p->d_cptr, p->d_cent, p->d_uptr, p->d_uent, p->d_pin,
                     p->d_rs, p->d_cs, p->d_lval, p->d_uval,
                     p->f64.blk, p->f32.blk, p->d_ipiv, p->d_info, p->d_v, p->d_t, p->d_w,
-                    p->d_null[0], p->d_null[1], p->d_unit, p->d_vm, p->d_tm, p->d_mmw, p->d_pin_rows};
+                    p->d_null[0], p->d_null[1], p->d_unit, p->d_vm, p->d_tm, p->d_mmw, p->d_pin_rows, p->d_luw, p->d_rowperm};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->blas) rocblas_destroy_handle(p->blas);
@@ -499,6 +506,8 @@ static int rowlu_ensure(iemic_ctx *c)
     if (const char *v = std::getenv("IEMIC_DENSE_LU")) p->dense_lu = std::atoi(v) != 0;
     if (const char *v = std::getenv("IEMIC_BLAS_APPLY")) p->blas_apply = std::atoi(v) != 0;
     HIPCHK(hipMalloc(&p->d_mmw, dense_mm_work(p->nmax) * sizeof(double)));
+    HIPCHK(hipMalloc(&p->d_luw, dense_getrf_work_bytes(p->nmax)));
+    HIPCHK(hipMalloc(&p->d_rowperm, (size_t)std::max(p->nmax, 1) * sizeof(int)));
     BLASCHK(rocblas_create_handle(&p->blas));
     BLASCHK(rocblas_set_stream(p->blas, c->stream));
     // size the handle's workspace once for the largest row, so that the
@@ -512,6 +521,7 @@ static int rowlu_ensure(iemic_ctx *c)
         rocsolver_dgetrs(p->blas, rocblas_operation_none, p->nmax, p->nmax, dd, p->nmax, p->d_ipiv, dd, p->nmax);
         rocsolver_sgetrf(p->blas, p->nmax, p->nmax, df, p->nmax, p->d_ipiv, p->d_info);
         rocsolver_sgetrs(p->blas, rocblas_operation_none, p->nmax, p->nmax, df, p->nmax, p->d_ipiv, df, p->nmax);
+        dense_getrf_size_query(p->blas, p->nmax);
         BLASCHK(rocblas_stop_device_memory_size_query(p->blas, &ws));
         BLASCHK(rocblas_set_device_memory_size(p->blas, ws));
     }
@@ -596,17 +606,18 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
                                p->d_lent, p->d_lval, X, D);
         }
         HIPCHK(hipGetLastError());
+        // Minv_j: solve D W = I, then W replaces the factors
+        T *W = reinterpret_cast<T *>(p->d_w);
         if (p->dense_lu) {
-            int rc = dense_getrf<T>(p->blas, c->stream, nr, D, p->d_ipiv, p->d_info + j);
+            int rc = dense_getrf<T>(p->blas, c->stream, nr, D, p->d_ipiv, p->d_info + j, p->d_luw, p->d_rowperm);
+            if (!rc) rc = dense_inverse<T>(p->blas, c->stream, nr, D, p->d_rowperm, W);
             if (rc) return rc;
         } else {
             BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
+            hipLaunchKernelGGL(k_identity<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 8192)), dim3(256), 0,
+                               c->stream, W, nr);
+            BLASCHK(getrs(p->blas, nr, D, p->d_ipiv, W));
         }
-        // Minv_j: solve D W = I, then W replaces the factors
-        T *W = reinterpret_cast<T *>(p->d_w);
-        hipLaunchKernelGGL(k_identity<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 8192)), dim3(256), 0, c->stream, W,
-                           nr);
-        BLASCHK(getrs(p->blas, nr, D, p->d_ipiv, W));
         HIPCHK(hipMemcpyAsync(D, W, (size_t)nr * nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     }
     std::vector<int> info(p->m, 0);

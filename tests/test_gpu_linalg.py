@@ -129,14 +129,22 @@ def test_row_lu_solve_fp64(name):
     assert info.iterations <= 30
 
 
-@pytest.mark.parametrize("name", ["natl8", "box16", "gateway16"])
-def test_row_lu_is_exact_for_the_factored_matrix(name):
+@pytest.mark.parametrize("lu", ["rocsolver", "tournament"])
+@pytest.mark.parametrize("name", ["natl8", "box16", "gateway16", "global96"])
+def test_row_lu_is_exact_for_the_factored_matrix(name, lu):
     """M^{-1} is an exact inverse of J except in the rows the factored matrix
     replaces (one continuity row per pressure basin, the integral row):
-    J M^{-1} v = v on every other row."""
+    J M^{-1} v = v on every other row.  Both dense LUs of the row blocks:
+    rocSOLVER getrf (partial pivoting) and the library's tournament-pivoted
+    LU (dense_lu.hip)."""
     case = _case(name)
-    oc = gpu_ocean(case)
-    oc.computeJacobian(case.x)
+    os.environ["IEMIC_DENSE_LU"] = "1" if lu == "tournament" else "0"
+    try:
+        oc = gpu_ocean(case)
+        oc.computeJacobian(case.x)
+        oc.buildPreconditioner()
+    finally:
+        del os.environ["IEMIC_DENSE_LU"]
     pins = oc.preconditionerInfo().pins
     J = _csr(oc)
     v = J @ np.random.default_rng(3).normal(size=oc.ndim)
@@ -152,11 +160,12 @@ def test_row_lu_is_exact_for_the_factored_matrix(name):
 
 
 @pytest.mark.parametrize("name", ["gateway16", "global96"])
-def test_row_lu_with_own_dense_lu(name):
-    """The row blocks factored by the library's own panel LU (dense_lu.hip)
-    instead of rocSOLVER getrf: the same FP64 solve bar."""
+def test_row_lu_with_rocsolver_lu(name):
+    """The row blocks factored by rocSOLVER getrf (partial pivoting) and
+    inverted by getrs instead of the library's tournament-pivoted LU: the
+    same FP64 solve bar."""
     case = _case(name)
-    os.environ["IEMIC_DENSE_LU"] = "1"
+    os.environ["IEMIC_DENSE_LU"] = "0"
     try:
         oc = gpu_ocean(case)
         F = oc.computeRHS(case.x)
