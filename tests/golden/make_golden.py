@@ -79,8 +79,35 @@ def cases():
     landm = RefModel.reference_landm(cfg, 0, 1, "mask_global_96x38x12")
     yield Case("global96", cfg, landm, analytic_forcing(cfg),
                dict(COMB=1.0, WIND=1.0, SALT=1.0, TEMP=1.0), rand_state(cfg, 6)), 0.02
+    yield reft16_case(), 0.25
+
+
+def reft16_case():
+    """The configuration of the reference test reft_ocean
+    (test/ocean/reft_ocean_params.xml: 16x16x16 gateway mask over
+    300-340E x 20-60N, periodic, topography from the mask, idealised
+    wind/temperature/salinity forcing of Forcing Type 2, restoring T and S)
+    at the steady state that the test stores (test/ocean/ocean_reference.h5,
+    converted to reft/ocean_reference_state.npz by make_reft_state.py), with the stored
+    parameters.  The test runs Mixing = 2, whose right-hand side at this state
+    is bitwise the one of Mixing = 1 (vmix_control switches both mixing terms
+    on once T and S are nonzero, mix_imp.f:131-166; only the grouping of the
+    finite-difference mixing Jacobian differs, vmix_el_2), so the fixture uses
+    Mixing = 1, the form the GPU library implements."""
+    from thcm_cases import PAR
+    z = np.load(os.path.join(HERE, "reft", "ocean_reference_state.npz"), allow_pickle=False)
+    cfg = make_config(n=16, m=16, l=16, xmin_deg=300.0, xmax_deg=340.0, ymin_deg=20.0, ymax_deg=60.0,
+                      periodic=1, hdim=4000.0, qz=1.0, vmix=1, tap=1, rho_mixing=0, tres=1, sres=1,
+                      ite=1, its=1, iza=2, forcing_type=2)
+    landm = RefModel.reference_landm(cfg, 0, 1, "mask_gateway")
+    assert np.array_equal(landm, z["mask"]), "mask_gateway differs from the stored MaskGlobal"
+    name = {v: k for k, v in PAR.items()}
+    pars = {name[i]: float(z["par"][i]) for i in range(1, 31)}
+    return Case("reft16", cfg, landm, {}, pars, z["x"])
 
 
 if __name__ == "__main__":
+    only = set(sys.argv[1:])   # case names to write (all when none are given)
     for case, frac in cases():
-        write(case, frac)
+        if not only or case.name in only:
+            write(case, frac)

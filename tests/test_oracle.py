@@ -141,3 +141,30 @@ def test_checkerboard_modes_span_the_null_space(name):
         null = Vt[S <= 1e-14 * S[0]]
         assert null.shape[0] == 2
         assert np.linalg.norm(null - (null @ s.T) @ s) <= 1e-8
+
+
+def test_stored_state_of_reft_ocean_is_a_steady_state():
+    """The steady state the reference's own end-to-end test stores
+    (test/ocean/ocean_reference.h5, kept as data in
+    tests/golden/reft/ocean_reference_state.npz) is one of the restated
+    reft_ocean configuration: F is 7.7e-6 in the 2-norm there with the
+    convective mixing on, against the 1e-2 Newton tolerance of
+    test/ocean/reft_continuation_params.xml, while with the mixing off the
+    same state leaves 0.157.  This pins the configuration (grid, mask,
+    forcing types, the 30 stored parameters) and shows that the reference's
+    Mixing = 2 run had its mixing terms on."""
+    import os
+    from thcm_cases import GOLDEN, Config, config_dict
+    case, exp = load_golden("reft16")
+    z = np.load(os.path.join(GOLDEN, "reft", "ocean_reference_state.npz"), allow_pickle=False)
+    np.testing.assert_array_equal(case.x, z["x"])
+    # the u, v, T, S norms reft_ocean.C:69-88 compares
+    np.testing.assert_allclose([np.linalg.norm(case.x[v::6]) for v in (0, 1, 4, 5)],
+                               [0.097907, 0.022403, 3.516206, 0.035162], atol=1e-6)
+    F = OracleModel(case).rhs()
+    np.testing.assert_array_equal(F, exp["F"])
+    assert np.linalg.norm(F) < 1e-5
+    cfg0 = Config(**config_dict(case.cfg))
+    cfg0.vmix = 0
+    F0 = OracleModel(Case("reft16_nomix", cfg0, case.landm, case.fields, case.pars, case.x)).rhs()
+    assert np.linalg.norm(F0) > 0.1
