@@ -125,6 +125,8 @@ def load():
         "iemic_apply_tuning": ([vp, C.c_int, ip], C.c_int),
         "iemic_spmm_tuning": ([vp, C.c_int, ip], C.c_int),
         "iemic_read_mask": ([C.c_char_p, C.c_int, C.c_int, C.c_int, ip], C.c_int),
+        "iemic_read_state": ([C.c_char_p, dp, C.c_int64, dp, ip], C.c_int),
+        "iemic_write_state": ([C.c_char_p, dp, C.c_int64, dp], C.c_int),
         "iemic_newton_step_many": ([vp, C.c_int, vp, vp, vp, vp, C.c_int], C.c_int),
         "iemic_continuation_dev": ([vp, vp, vp, vp, dp, dp, vp], C.c_int),
         "iemic_pressure_projection_dev": ([vp, vp], C.c_int),
@@ -195,6 +197,34 @@ def read_mask(path, n, m, l):
     out = np.zeros((n + 2) * (m + 2) * (l + 2), np.int32)
     _check(load().iemic_read_mask(str(path).encode(), n, m, l, out.ctypes.data_as(_P(C.c_int))))
     return out
+
+
+def read_state(path, ndim):
+    """A state file in the reference's EpetraExt-HDF5 layout
+    (iemic_read_state, after Model::loadStateFromFile): returns (x, pars)
+    with pars {par.F90 index: value} for the parameters the file holds."""
+    x = np.zeros(ndim)
+    par = np.zeros(31)
+    found = np.zeros(31, np.int32)
+    _check(load().iemic_read_state(str(path).encode(), x.ctypes.data_as(_P(C.c_double)), ndim,
+                                   par.ctypes.data_as(_P(C.c_double)), found.ctypes.data_as(_P(C.c_int))))
+    return x, {i: float(par[i]) for i in range(1, 31) if found[i]}
+
+
+def write_state(path, x, pars=None):
+    """Writes x and the parameters {par.F90 index: value} (all 30, or none)
+    in the reference's state-file layout (iemic_write_state, after
+    Model::saveStateToFile)."""
+    x = np.ascontiguousarray(x, np.float64)
+    pp = None
+    if pars is not None:
+        if sorted(pars) != list(range(1, 31)):
+            raise ValueError("write_state needs all 30 parameters (par.F90 indices 1..30)")
+        arr = np.zeros(31)
+        for i, v in pars.items():
+            arr[i] = v
+        pp = arr.ctypes.data_as(_P(C.c_double))
+    _check(load().iemic_write_state(str(path).encode(), x.ctypes.data_as(_P(C.c_double)), x.size, pp))
 
 
 class DeviceArray:
@@ -283,6 +313,20 @@ class Ocean:
         v = C.c_double()
         _check(self.lib.iemic_get_par(self.h, self._idx(name), C.byref(v)))
         return v.value
+
+    def loadStateFromFile(self, path):
+        """Model::loadStateFromFile (Model.C:15-110): the state of a
+        reference state file, with the parameters the file holds set on this
+        model; returns the state (host array)."""
+        x, pars = read_state(path, self.ndim)
+        for i, v in pars.items():
+            self.setPar(i, v)
+        return x
+
+    def saveStateToFile(self, path, x):
+        """Model::saveStateToFile (Model.C:112-160): x and this model's 30
+        parameters in the reference's state-file layout."""
+        write_state(path, x, {i: self.getPar(i) for i in range(1, 31)})
 
     # ---- structure
     def rowintcon(self):

@@ -25,6 +25,8 @@
 #ifndef IEMIC_AMD_H
 #define IEMIC_AMD_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -36,7 +38,8 @@ enum {
     IEMIC_EHIP = -3,       /* HIP runtime error                         */
     IEMIC_ENOMEM = -4,     /* device allocation failed                  */
     IEMIC_ESTATE = -5,     /* call order (e.g. no Jacobian yet)         */
-    IEMIC_ENOCONV = -6     /* solver did not reach its tolerance        */
+    IEMIC_ENOCONV = -6,    /* solver did not reach its tolerance        */
+    IEMIC_EIO = -7         /* a file could not be read or written       */
 };
 
 /* THCM initialisation parameters (THCM::THCM, THCM.C:180-260, and the
@@ -278,6 +281,24 @@ int iemic_newton_step_many(iemic_ctx *const *ctx, int count, const double *const
  * only (no GPU needed).  Rows may hold at most 362 values, as the reference
  * format does. */
 int iemic_read_mask(const char *path, int n, int m, int l, int *landm);
+
+/* ---- state files (EpetraExt-HDF5, Model.C:15-160) ------------------------ */
+/* Model::loadStateFromFile (Model.C:15-110): reads /State/Values of a state
+ * file in the reference's layout (/State: Values [1][n], GlobalLength,
+ * NumVectors, __type__; /Parameters: one double per THCM::int2par label,
+ * THCM.C:1895-1937) into x (n doubles, reference ordering).  par and found
+ * (31 entries each, index 1..30 as par.F90; either may be NULL) receive the
+ * parameters the file holds and 1/0 for present/absent (the reference skips
+ * absent ones).  Fails with IEMIC_EINVAL if the file's length differs from n.
+ * Host only; the HDF5 C library is loaded at the first call (libhdf5.so,
+ * /opt/conda/lib/libhdf5.so, or $IEMIC_HDF5_LIB), IEMIC_EUNSUPPORTED if none
+ * is found. */
+int iemic_read_state(const char *path, double *x, int64_t n, double *par, int *found);
+/* Model::saveStateToFile (Model.C:112-160): writes x (n doubles) and the 30
+ * parameters par[1..30] (may be NULL: no /Parameters group) in that layout,
+ * creating or truncating path.  The /Grid group the reference adds for its
+ * plotting scripts is not written. */
+int iemic_write_state(const char *path, const double *x, int64_t n, const double *par);
 
 /* ---- device memory helpers for callers without their own allocator ----- */
 int iemic_dev_alloc(void **ptr, long long bytes);

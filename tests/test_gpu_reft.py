@@ -75,3 +75,18 @@ def test_gpu_reproduces_reft_ocean():
     assert info.reached
     assert abs(info.par - 0.02) <= 1e-4   # the test's destination tolerance
     assert np.all(np.abs(got - want) <= 1e-3)
+
+
+def test_gpu_model_loads_the_reference_state_file():
+    """Ocean.loadStateFromFile (iemic_read_state, after
+    Model::loadStateFromFile) on the reference's own file: a model built at
+    rest (COMB = 0) takes the file's state and parameters, and F there
+    equals the fixture's (written from the reference Fortran) bit for bit."""
+    case, exp = load_golden("reft16")
+    pars = dict(case.pars)
+    pars["COMB"] = 0.0
+    oc = gpu_ocean(Case("reft16_rest", case.cfg, case.landm, case.fields, pars, np.zeros(case.ndim)))
+    x = oc.loadStateFromFile(os.path.join(GOLDEN, "reft", "ocean_reference.h5"))
+    assert oc.getPar("COMB") == 0.020000184160660548
+    np.testing.assert_array_equal(x, case.x)
+    np.testing.assert_array_equal(oc.computeRHS(x), exp["F"])
