@@ -2,7 +2,7 @@
 // (rowlu.hip), same contract as LAPACK getrf: A = P L U in place, L unit
 // lower, ipiv 1-based (row i was interchanged with row ipiv[i]), info = first
 // zero pivot (1-based) or 0, so LAPACK-style solvers (rocSOLVER getrs) apply
-// the factors as is; dense_inverse forms A^{-1} from them directly.
+// the factors as is; dense_inverse forms A^{-1} from them in place.
 //
 // Why not rocSOLVER getrf: its panel factorisation launches three kernels
 // per column (pivot search, scale/update, singularity check) of 2-4 us each,
@@ -296,16 +296,27 @@ __global__ void k_lu_iota(int *__restrict__ p, int n)
     if (i < n) p[i] = i;
 }
 
-// W = P^T (the identity with its rows permuted as LAPACK's interchanges
-// permute them): W[i, perm[i]] = 1, zero elsewhere.  Grid-stride over the
-// n x n elements.
+// After trtri of the unit lower factor: the strictly upper part of W
+// cleared and its diagonal set to 1 (trtri leaves both unspecified).
 template <typename T>
-__global__ void k_lu_perm_identity(T *__restrict__ W, int n, const int *__restrict__ perm)
+__global__ void k_lu_unit_lower(T *__restrict__ W, int n)
 {
     const size_t tot = (size_t)n * n;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(e % n), j = (int)(e / n);
-        W[e] = perm[i] == j ? (T)1 : (T)0;
+        if (i <= j) W[e] = i == j ? (T)1 : (T)0;
+    }
+}
+
+// A^{-1} = X P^T with X = U^{-1} L^{-1}: column k of X is column perm[k]
+// of the inverse.  Grid-stride over the n x n elements, column by column.
+template <typename T>
+__global__ void k_lu_scatter_cols(const T *__restrict__ X, T *__restrict__ out, int n, const int *__restrict__ perm)
+{
+    const size_t tot = (size_t)n * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % n), k = (int)(e / n);
+        out[i + (size_t)perm[k] * n] = X[e];
     }
 }
 
@@ -443,7 +454,9 @@ void dense_getrf_size_query(rocblas_handle h, int n)
         (void)trsm_llnu(h, CA_NB, n - CA_NB, df, n, df, n);
         (void)gemm_nn_sub(h, n - CA_NB, n - CA_NB, CA_NB, df, n, df, n, df, n);
     }
-    // the n x n solves of dense_inverse
+    // the trtri and the n x n solve of dense_inverse
+    (void)rocblas_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, dd, n, dd, n);
+    (void)rocblas_strtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, df, n, df, n);
     (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, n, n,
                         &one, dd, n, dd, n);
     (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, n,
@@ -532,24 +545,38 @@ static rocblas_status trsm_lunn(rocblas_handle h, int n, const float *U, float *
                          n, n, &one, U, n, B, n);
 }
 
-// W = A^{-1} from the factors of dense_getrf: W = U^{-1} L^{-1} P^T, the
-// permuted identity written directly (no row interchanges on W).
+static rocblas_status trtri_lower_unit(rocblas_handle h, int n, const double *L, double *W)
+{
+    return rocblas_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, L, n, W, n);
+}
+static rocblas_status trtri_lower_unit(rocblas_handle h, int n, const float *L, float *W)
+{
+    return rocblas_strtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, L, n, W, n);
+}
+
+// A^{-1} from the factors of dense_getrf, written over them: W = L^{-1}
+// (trtri, n^3/3 flops), W = U^{-1} W (triangular solve, n^3), then the
+// columns of W scattered by the row permutation (A^{-1} = U^{-1} L^{-1} P^T).
+// Against two triangular solves with the permuted identity (2 n^3) this
+// saves a third of the inverse's flops.  W is n x n workspace.
 template <typename T>
-int dense_inverse(rocblas_handle h, hipStream_t s, int n, const T *LU, const int *perm, T *W)
+int dense_inverse(rocblas_handle h, hipStream_t s, int n, T *LU, const int *perm, T *W)
 {
     if (n <= 0) return IEMIC_OK;
     const size_t tot = (size_t)n * n;
-    hipLaunchKernelGGL(k_lu_perm_identity<T>, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 8192)), dim3(256), 0,
-                       s, W, n, perm);
+    const unsigned grid = (unsigned)std::min<size_t>((tot + 255) / 256, 8192);
+    LU_BLASCHK(trtri_lower_unit(h, n, LU, W));
+    hipLaunchKernelGGL(k_lu_unit_lower<T>, dim3(grid), dim3(256), 0, s, W, n);
     HIPCHK(hipGetLastError());
-    LU_BLASCHK(trsm_llnu(h, n, n, LU, n, W, n));
     LU_BLASCHK(trsm_lunn(h, n, LU, W));
+    hipLaunchKernelGGL(k_lu_scatter_cols<T>, dim3(grid), dim3(256), 0, s, W, LU, n, perm);
+    HIPCHK(hipGetLastError());
     return IEMIC_OK;
 }
 
 template int dense_getrf<double>(rocblas_handle, hipStream_t, int, double *, int *, int *, void *, int *);
 template int dense_getrf<float>(rocblas_handle, hipStream_t, int, float *, int *, int *, void *, int *);
-template int dense_inverse<double>(rocblas_handle, hipStream_t, int, const double *, const int *, double *);
-template int dense_inverse<float>(rocblas_handle, hipStream_t, int, const float *, const int *, float *);
+template int dense_inverse<double>(rocblas_handle, hipStream_t, int, double *, const int *, double *);
+template int dense_inverse<float>(rocblas_handle, hipStream_t, int, float *, const int *, float *);
 
 }  // namespace iemic

@@ -49,7 +49,7 @@ std::vector<int> column_tops(const iemic_ctx *c);
 template <typename T>
 int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info, void *work, int *perm);
 template <typename T>
-int dense_inverse(rocblas_handle h, hipStream_t s, int n, const T *LU, const int *perm, T *W);
+int dense_inverse(rocblas_handle h, hipStream_t s, int n, T *LU, const int *perm, T *W);
 size_t dense_getrf_work_bytes(int n);
 void dense_getrf_size_query(rocblas_handle h, int n);
 template <typename T>
@@ -608,16 +608,16 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
         HIPCHK(hipGetLastError());
         // Minv_j: solve D W = I, then W replaces the factors
         T *W = reinterpret_cast<T *>(p->d_w);
-        if (p->dense_lu) {
+        if (p->dense_lu) {   // the inverse replaces the factors in D
             int rc = dense_getrf<T>(p->blas, c->stream, nr, D, p->d_ipiv, p->d_info + j, p->d_luw, p->d_rowperm);
             if (!rc) rc = dense_inverse<T>(p->blas, c->stream, nr, D, p->d_rowperm, W);
             if (rc) return rc;
-        } else {
-            BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
-            hipLaunchKernelGGL(k_identity<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 8192)), dim3(256), 0,
-                               c->stream, W, nr);
-            BLASCHK(getrs(p->blas, nr, D, p->d_ipiv, W));
+            continue;
         }
+        BLASCHK(getrf(p->blas, nr, D, p->d_ipiv, p->d_info + j));
+        hipLaunchKernelGGL(k_identity<T>, dim3(std::min(nblk((size_t)nr * nr, 256), 8192)), dim3(256), 0, c->stream, W,
+                           nr);
+        BLASCHK(getrs(p->blas, nr, D, p->d_ipiv, W));
         HIPCHK(hipMemcpyAsync(D, W, (size_t)nr * nr * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     }
     std::vector<int> info(p->m, 0);
