@@ -10,7 +10,7 @@
 // preconditioner build.  Here the pivots of a whole sub-panel of CA_W
 // columns are chosen at once by tournament pivoting (communication-avoiding
 // LU, Grigori, Demmel and Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)):
-//   1. every group of CA_G rows runs Gaussian elimination with partial
+//   1. every group of (up to) CA_G rows runs Gaussian elimination with partial
 //      pivoting on its rows of the sub-panel, in registers, and nominates the
 //      CA_W rows it pivots on;
 //   2. the nominees are merged CA_FAN sets at a time by the same elimination
@@ -41,7 +41,7 @@
 namespace iemic {
 
 constexpr int CA_W = 32;               // sub-panel width
-constexpr int CA_G = 512;              // rows per selection group (one row per thread)
+constexpr int CA_G = 512;              // rows per selection group (one row per thread), at most
 constexpr int CA_FAN = CA_G / CA_W;    // candidate sets merged per group above the first level
 constexpr int CA_NB = 256;             // outer block width (TRSM/GEMM depth)
 constexpr int CA_MOVES = 2 * CA_W;     // rows a sub-panel's interchanges can touch
@@ -110,26 +110,29 @@ __device__ __forceinline__ int wave_argmax_abs(T v, bool live, unsigned long lon
 // Gaussian elimination with partial pivoting on the rows of one group,
 // restricted to the sub-panel columns [k, k + w): nominates up to w rows, in
 // pivot order (largest magnitude; on ties the group's first row).  The group's
-// rows are rows_in[g CA_G + t] (absolute row indices, -1 = none), or
-// k + g CA_G + t when rows_in is null.  At the final level (one group) the
+// rows are rows_in[g G + t] (absolute row indices, -1 = none), or
+// k + g G + t when rows_in is null; G rows per group (64 to CA_G, one per
+// thread; a level that fits one group runs in the smallest G that holds it,
+// since a column step of a 64-row group takes about half the time of a
+// 512-row group's).  At the final level (one group) the
 // workgroup also writes the factors of the chosen rows (lu11, column-major,
 // leading dimension CA_W), the LAPACK interchanges ipiv[k .. k + w), the move
 // list that performs them, and info at the first zero pivot.
-template <typename T>
-__global__ void __launch_bounds__(CA_G)
+template <typename T, int G>
+__global__ void __launch_bounds__(G)
     k_calu_select(const T *__restrict__ A, int lda, int k, int w, const int *__restrict__ rows_in, int cnt,
                   int *__restrict__ rows_out, int final_level, T *__restrict__ lu11, int *__restrict__ ipiv,
                   int *__restrict__ moves, int *__restrict__ info)
 {
     // per wave and column parity: the wave's best row (its key, thread, and
     // values), so that one barrier per column suffices
-    __shared__ T wrow[2][CA_G / 64][CA_W];
-    __shared__ unsigned long long wk[2][CA_G / 64];
-    __shared__ int wt[2][CA_G / 64];
+    __shared__ T wrow[2][G / 64][CA_W];
+    __shared__ unsigned long long wk[2][G / 64];
+    __shared__ int wt[2][G / 64];
     __shared__ T s_u[CA_W][CA_W + 1];   // rows of U11 as they are chosen
     __shared__ int s_sel[CA_W];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int idx = blockIdx.x * CA_G + t;
+    const int idx = blockIdx.x * G + t;
     const int myrow = idx < cnt ? (rows_in ? rows_in[idx] : k + idx) : -1;
     bool live = myrow >= 0;
     // The row stays in registers.  The elimination updates all 32 columns
@@ -167,7 +170,7 @@ __global__ void __launch_bounds__(CA_G)
             unsigned long long bk = wk[par][0];
             int bw = 0;
 #pragma unroll
-            for (int q = 1; q < CA_G / 64; q++)
+            for (int q = 1; q < G / 64; q++)
                 if (wk[par][q] > bk) {
                     bk = wk[par][q];
                     bw = q;
@@ -439,6 +442,38 @@ static rocblas_status gemm_nn_sub(rocblas_handle h, int m, int n, int k, const f
     return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &mone, A, lda, B, ldb, &one, C, ldc);
 }
 
+// Rows per selection group for a level of cnt rows: the smallest of 64,
+// 128, 256 that holds them all, else CA_G (several groups).
+static int select_group(int cnt)
+{
+    for (int g = 64; g < CA_G; g *= 2)
+        if (cnt <= g) return g;
+    return CA_G;
+}
+
+template <typename T>
+static void launch_select(int g, int groups, hipStream_t s, const T *A, int lda, int k, int w, const int *in, int cnt,
+                          int *out, int fin, T *lu11, int *ipiv, int *moves, int *info)
+{
+    switch (g) {
+    case 64:
+        hipLaunchKernelGGL((k_calu_select<T, 64>), dim3(groups), dim3(64), 0, s, A, lda, k, w, in, cnt, out, fin, lu11,
+                           ipiv, moves, info);
+        break;
+    case 128:
+        hipLaunchKernelGGL((k_calu_select<T, 128>), dim3(groups), dim3(128), 0, s, A, lda, k, w, in, cnt, out, fin,
+                           lu11, ipiv, moves, info);
+        break;
+    case 256:
+        hipLaunchKernelGGL((k_calu_select<T, 256>), dim3(groups), dim3(256), 0, s, A, lda, k, w, in, cnt, out, fin,
+                           lu11, ipiv, moves, info);
+        break;
+    default:
+        hipLaunchKernelGGL((k_calu_select<T, CA_G>), dim3(groups), dim3(CA_G), 0, s, A, lda, k, w, in, cnt, out, fin,
+                           lu11, ipiv, moves, info);
+    }
+}
+
 // Inside rocblas_start/stop_device_memory_size_query: the rocBLAS calls of
 // dense_getrf at their largest shapes for an n x n block, so that the
 // handle's workspace covers them.
@@ -494,18 +529,17 @@ int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *in
         for (int k = K; k < K + nb; k += CA_W) {
             const int w = std::min(CA_W, K + nb - k);
             // tournament over rows [k, n)
-            int cnt = n - k, groups = (cnt + CA_G - 1) / CA_G;
+            int cnt = n - k;
             const int *in = nullptr;
             int lev = 0;
             for (;;) {
+                const int g = select_group(cnt), groups = (cnt + g - 1) / g;
                 const int fin = groups == 1;
                 int *out = cand[lev & 1];
-                hipLaunchKernelGGL(k_calu_select<T>, dim3(groups), dim3(CA_G), 0, s, A, n, k, w, in, cnt, out, fin,
-                                   lu11, ipiv, moves, info);
+                launch_select<T>(g, groups, s, A, n, k, w, in, cnt, out, fin, lu11, ipiv, moves, info);
                 if (fin) break;
                 in = out;
                 cnt = groups * CA_W;
-                groups = (cnt + CA_G - 1) / CA_G;
                 lev++;
             }
             hipLaunchKernelGGL(k_calu_swap<T>, dim3((n + 1 + 3) / 4), dim3(256), 0, s, A, n, n, moves, perm);
