@@ -331,11 +331,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_calu_l21(T *__restrict__ A, int lda, int n, int k, int w, int c1,
                                                   const T *__restrict__ lu11, int nrowblk)
 {
-    __shared__ T s_lu[CA_W][CA_W + 1];
     __shared__ T s_rd[CA_W];
-    __shared__ T s_b[CA_W][CA_STRIP + 1];
     const int t = threadIdx.x;
-    for (int e = t; e < w * w; e += blockDim.x) s_lu[e % w][e / w] = lu11[(e % w) + (e / w) * CA_W];
     if (t < w) s_rd[t] = (T)1 / lu11[t + t * CA_W];
     const int b = blockIdx.x;
     if (b < nrowblk) {
@@ -345,7 +342,10 @@ __global__ void __launch_bounds__(256) k_calu_l21(T *__restrict__ A, int lda, in
         __syncthreads();
         const int r = k + w + b * blockDim.x + t;
         if (r >= n) return;
-        // x U11 = a, column by column: x_c = a_c / u_cc, then a_q -= x_c u_cq
+        // x U11 = a, column by column: x_c = a_c / u_cc, then a_q -= x_c u_cq.
+        // U11 is read from global memory at uniform addresses (scalar loads
+        // into scalar registers); from LDS every one of the ~500 products
+        // would be a broadcast LDS read.
         T a[CA_W];
 #pragma unroll
         for (int c = 0; c < CA_W; c++) a[c] = c < w ? A[r + (size_t)(k + c) * lda] : (T)0;
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(256) k_calu_l21(T *__restrict__ A, int lda, in
                 const T x = a[c] * s_rd[c];
                 a[c] = x;
 #pragma unroll
-                for (int q = c + 1; q < CA_W; q++) a[q] -= x * s_lu[c][q];
+                for (int q = c + 1; q < CA_W; q++) a[q] = fma(-x, lu11[c + q * CA_W], a[q]);
             }
         }
 #pragma unroll
@@ -363,25 +363,27 @@ __global__ void __launch_bounds__(256) k_calu_l21(T *__restrict__ A, int lda, in
             if (c < w) A[r + (size_t)(k + c) * lda] = a[c];
         return;
     }
+    // U12 = L11^{-1} A12: one column per thread, held in registers;
+    // column-oriented substitution with the unit lower L11 read at uniform
+    // addresses (a serial per-column loop over an LDS copy was this
+    // kernel's critical path)
     const int cs = k + w + (b - nrowblk) * CA_STRIP;
     const int nc = min(CA_STRIP, c1 - cs);
-    for (int e = t; e < w * CA_STRIP; e += blockDim.x) {
-        const int i = e % w, cc = e / w;
-        s_b[i][cc] = cc < nc ? A[(k + i) + (size_t)(cs + cc) * lda] : (T)0;
-    }
-    __syncthreads();
-    if (t < nc) {
-        for (int i = 1; i < w; i++) {
-            T s = s_b[i][t];
-            for (int p = 0; p < i; p++) s -= s_lu[i][p] * s_b[p][t];
-            s_b[i][t] = s;
+    if (t >= nc) return;
+    const size_t col = (size_t)(cs + t) * lda;
+    T x[CA_W];
+#pragma unroll
+    for (int i = 0; i < CA_W; i++) x[i] = i < w ? A[(k + i) + col] : (T)0;
+#pragma unroll
+    for (int p = 0; p < CA_W; p++) {
+        if (p < w) {
+#pragma unroll
+            for (int i = p + 1; i < CA_W; i++) x[i] = fma(-lu11[i + p * CA_W], x[p], x[i]);
         }
     }
-    __syncthreads();
-    for (int e = t; e < w * CA_STRIP; e += blockDim.x) {
-        const int i = e % w, cc = e / w;
-        if (cc < nc) A[(k + i) + (size_t)(cs + cc) * lda] = s_b[i][cc];
-    }
+#pragma unroll
+    for (int i = 0; i < CA_W; i++)
+        if (i < w) A[(k + i) + col] = x[i];
 }
 
 // Rank-w update of the block columns [c0, c1) on rows [k + w, n):
@@ -409,7 +411,7 @@ __global__ void __launch_bounds__(256) k_calu_update(T *__restrict__ A, int lda,
         for (int p = 0; p < w; p++) {
             const T l = A[r + (size_t)(k + p) * lda];
 #pragma unroll
-            for (int cc = 0; cc < CA_STRIP; cc++) acc[cc] -= l * U[p][cc];
+            for (int cc = 0; cc < CA_STRIP; cc++) acc[cc] = fma(-l, U[p][cc], acc[cc]);
         }
 #pragma unroll
         for (int cc = 0; cc < CA_STRIP; cc++)
