@@ -229,22 +229,26 @@ hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, si
 }
 
 // Y = alpha A^T X + beta Y with A column-major (m x k): output row c is the
-// dot product of column c of A (contiguous) with each column of X.  One
-// wavefront per output row walks its column 64 rows at a time; the four
-// wavefronts of a block share the chunk of X staged in LDS.
+// dot product of column c of A (contiguous) with each column of X.  Each
+// wavefront owns CW output rows and walks their columns 64 rows at a time;
+// the four wavefronts of a block share the chunk of X staged in LDS, and a
+// wavefront reads each staged X value once for its CW columns (CW = 4 cuts
+// the LDS reads and the X staging per output row fourfold when NC >= 4).
 constexpr int MT_CHUNK = 1024;   // rows of X staged per step
 
-template <typename T, int NC>
+template <typename T, int NC, int CW>
 __global__ void __launch_bounds__(256) k_mmt(int m, int k, int q, const T *__restrict__ A, size_t lda,
                                              const T *__restrict__ X, size_t ldx, T alpha, T beta,
                                              T *__restrict__ Y, size_t ldy)
 {
     __shared__ T xs[NC][MT_CHUNK];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int c = blockIdx.x * 4 + wave;
-    T acc[NC];
+    const int c0 = (blockIdx.x * 4 + wave) * CW;
+    T acc[CW][NC];
 #pragma unroll
-    for (int j = 0; j < NC; j++) acc[j] = T(0);
+    for (int u = 0; u < CW; u++)
+#pragma unroll
+        for (int j = 0; j < NC; j++) acc[u][j] = T(0);
     for (int r0 = 0; r0 < m; r0 += MT_CHUNK) {
         const int nr = min(MT_CHUNK, m - r0);
         for (int e = threadIdx.x; e < NC * MT_CHUNK; e += 256) {
@@ -252,29 +256,38 @@ __global__ void __launch_bounds__(256) k_mmt(int m, int k, int q, const T *__res
             xs[j][rr] = (rr < nr && j < q) ? X[(size_t)j * ldx + r0 + rr] : T(0);
         }
         __syncthreads();
-        if (c < k) {
-            const T *a = A + (size_t)c * lda + r0;
-            for (int rr = lane; rr < nr; rr += 64) {
-                const T av = a[rr];
+        for (int rr = lane; rr < nr; rr += 64) {
+            T av[CW];
 #pragma unroll
-                for (int j = 0; j < NC; j++) acc[j] += av * xs[j][rr];
+            for (int u = 0; u < CW; u++) av[u] = c0 + u < k ? A[(size_t)(c0 + u) * lda + r0 + rr] : T(0);
+#pragma unroll
+            for (int j = 0; j < NC; j++) {
+                const T xv = xs[j][rr];
+#pragma unroll
+                for (int u = 0; u < CW; u++) acc[u][j] += av[u] * xv;
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j < NC; j++) {
-        T v = acc[j];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        acc[j] = v;
-    }
-    if (c < k && lane < NC && lane < q) {
-        T v = acc[0];
+    for (int u = 0; u < CW; u++)
 #pragma unroll
-        for (int j = 1; j < NC; j++)
-            if (lane == j) v = acc[j];
-        T *y = Y + (size_t)lane * ldy + c;
-        *y = beta == T(0) ? alpha * v : alpha * v + beta * *y;
+        for (int j = 0; j < NC; j++) {
+            T v = acc[u][j];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            acc[u][j] = v;
+        }
+#pragma unroll
+    for (int u = 0; u < CW; u++) {
+        const int c = c0 + u;
+        if (c < k && lane < NC && lane < q) {
+            T v = acc[u][0];
+#pragma unroll
+            for (int j = 1; j < NC; j++)
+                if (lane == j) v = acc[u][j];
+            T *y = Y + (size_t)lane * ldy + c;
+            *y = beta == T(0) ? alpha * v : alpha * v + beta * *y;
+        }
     }
 }
 
@@ -285,11 +298,11 @@ hipError_t dense_mm_t(hipStream_t st, int m, int k, int q, T alpha, const T *A, 
 {
     if (k <= 0 || q <= 0) return hipSuccess;
     if (q > 8) return hipErrorInvalidValue;
-    const dim3 g((k + 3) / 4), b(256);
-    if (q == 1) hipLaunchKernelGGL((k_mmt<T, 1>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
-    else if (q <= 2) hipLaunchKernelGGL((k_mmt<T, 2>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
-    else if (q <= 4) hipLaunchKernelGGL((k_mmt<T, 4>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
-    else hipLaunchKernelGGL((k_mmt<T, 8>), g, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    const dim3 g1((k + 3) / 4), g4((k + 15) / 16), b(256);
+    if (q == 1) hipLaunchKernelGGL((k_mmt<T, 1, 1>), g1, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    else if (q <= 2) hipLaunchKernelGGL((k_mmt<T, 2, 1>), g1, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    else if (q <= 4) hipLaunchKernelGGL((k_mmt<T, 4, 4>), g4, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
+    else hipLaunchKernelGGL((k_mmt<T, 8, 4>), g4, b, 0, st, m, k, q, A, lda, X, ldx, alpha, beta, Y, ldy);
     return hipGetLastError();
 }
 

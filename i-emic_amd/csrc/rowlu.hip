@@ -984,11 +984,23 @@ static int rowlu_sweep_t_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, c
     if (p->nocean == 0) return IEMIC_OK;
     hipLaunchKernelGGL(k_rowm_gather<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean, p->d_perm,
                        p->d_cs, in, ldin, v);
+    // Y = alpha A^T X + beta Y (A column-major m x k): the library's
+    // transposed product, up to 8 columns per pass over A, or rocBLAS
+    auto mmt = [&](int m, int k, T alpha, const T *A, const T *X, size_t ldx, T beta, T *Y, size_t ldy) -> int {
+        if (p->blas_apply) {
+            BLASCHK(gemm_t(p->blas, k, nrhs, m, alpha, A, m, X, (int)ldx, beta, Y, (int)ldy));
+            return IEMIC_OK;
+        }
+        for (int c0 = 0; c0 < nrhs; c0 += 8)
+            HIPCHK(dense_mm_t<T>(c->stream, m, k, std::min(8, nrhs - c0), alpha, A, m, X + c0 * ldx, ldx, beta,
+                                 Y + c0 * ldy, ldy));
+        return IEMIC_OK;
+    };
     for (int j = 0; j + 1 < p->m; j++) {
         const int nr = p->nj[j], nn = p->nj[j + 1];
         if (nr == 0 || nn == 0) continue;
-        BLASCHK(gemm_t(p->blas, nn, nrhs, nr, (T)-1, F.blk + p->xoff[j + 1], nr, v + p->ubase[j], (int)ldv, (T)1,
-                       v + p->ubase[j + 1], (int)ldv));
+        if (int rc = mmt(nr, nn, (T)-1, F.blk + p->xoff[j + 1], v + p->ubase[j], ldv, (T)1, v + p->ubase[j + 1], ldv))
+            return rc;
     }
     for (int j = p->m - 1; j >= 0; j--) {
         const int nr = p->nj[j];
@@ -1000,7 +1012,7 @@ static int rowlu_sweep_t_multi(iemic_ctx *c, const RowFactors<T> &F, int nrhs, c
         else
             HIPCHK(hipMemcpy2DAsync(t, (size_t)nr * sizeof(T), vj, ldv * sizeof(T), (size_t)nr * sizeof(T), nrhs,
                                     hipMemcpyDeviceToDevice, c->stream));
-        BLASCHK(gemm_t(p->blas, nr, nrhs, nr, (T)1, F.blk + p->doff[j], nr, t, nr, (T)0, vj, (int)ldv));
+        if (int rc = mmt(nr, nr, (T)1, F.blk + p->doff[j], t, nr, (T)0, vj, ldv)) return rc;
     }
     hipLaunchKernelGGL(k_rowm_scatter<T>, dim3(nblk(p->nocean, 256), nrhs), dim3(256), 0, c->stream, p->nocean,
                        p->d_perm, p->d_rs, v, out, ldout);

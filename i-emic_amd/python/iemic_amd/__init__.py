@@ -384,6 +384,9 @@ class Ocean:
         _check(self.lib.iemic_apply_matrix(self.h, _d(v), _d(out)))
         return out
 
+    def applyMatrixTranspose_dev(self, dv, dout):
+        _check(self.lib.iemic_apply_matrix_transpose_dev(self.h, dv.ptr, dout.ptr))
+
     def applyMatrixTranspose(self, v):
         dv = DeviceArray(self.ndim).upload(v)
         do = DeviceArray(self.ndim)
@@ -475,10 +478,12 @@ class Ocean:
     def applyPrecon_dev(self, dv, dout):
         _check(self.lib.iemic_apply_precon_dev(self.h, dv.ptr, dout.ptr))
 
-    def solve_dev(self, db, dx, params=None, raise_on_noconv=True):
+    def solve_dev(self, db, dx, params=None, raise_on_noconv=True, transpose=False):
+        """J x = b (J^T x = b if transpose) on device arrays."""
         info = SolveInfo()
         p = params if params is not None else self.solver_params()
-        rc = self.lib.iemic_solve_dev(self.h, C.byref(p), db.ptr, dx.ptr, C.byref(info))
+        fn = self.lib.iemic_solve_transpose_dev if transpose else self.lib.iemic_solve_dev
+        rc = fn(self.h, C.byref(p), db.ptr, dx.ptr, C.byref(info))
         if rc != 0 and (raise_on_noconv or rc != -6):
             _check(rc)
         return info
@@ -547,10 +552,13 @@ class Ocean:
             _check(rc)
         return Xc.T.copy(), list(infos)
 
-    def solve_multi_dev(self, dB, dX, nrhs, params=None, raise_on_noconv=True):
+    def solve_multi_dev(self, dB, dX, nrhs, params=None, raise_on_noconv=True, transpose=False):
+        """J X = B (J^T X = B if transpose) for nrhs columns stored one after
+        another in device arrays."""
         infos = (SolveInfo * nrhs)()
         p = params if params is not None else self.solver_params()
-        rc = self.lib.iemic_solve_multi_dev(self.h, C.byref(p), nrhs, dB.ptr, dX.ptr, infos)
+        fn = self.lib.iemic_solve_multi_transpose_dev if transpose else self.lib.iemic_solve_multi_dev
+        rc = fn(self.h, C.byref(p), nrhs, dB.ptr, dX.ptr, infos)
         if rc != 0 and (raise_on_noconv or rc != -6):
             _check(rc)
         return list(infos)
