@@ -30,7 +30,6 @@
 // three per column.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <climits>
@@ -509,12 +508,8 @@ void dense_getrf_size_query(rocblas_handle h, int n)
     // the trtri and the n x n solve of dense_inverse
     (void)rocblas_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, dd, n, dd, n);
     (void)rocblas_strtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, df, n, df, n);
-    (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, n, n,
-                        &one, dd, n, dd, n);
     (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, n,
                         n, &one, dd, n, dd, n);
-    (void)rocblas_strsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, n, n,
-                        &onef, df, n, df, n);
     (void)rocblas_strsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit, n,
                         n, &onef, df, n, df, n);
 }
