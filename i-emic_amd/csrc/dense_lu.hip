@@ -16,9 +16,8 @@
 //   2. the nominees are merged CA_FAN sets at a time by the same elimination
 //      on their (original) rows until one set is left;
 //   3. those rows are moved to the top, in the order chosen, and the
-//      sub-panel is factored without further pivoting: U11 is what the last
-//      elimination computed, L11 follows from the chosen rows by one
-//      triangular solve against U11, and L21 = A21 U11^{-1}.
+//      sub-panel is factored without further pivoting: U11 and L11 are what
+//      the last elimination computed, and L21 = A21 U11^{-1}.
 // The pivot rows of a panel then span its column space whenever the panel
 // has full column rank, and the element growth stays close to partial
 // pivoting in practice; only the pivot order differs from LAPACK's.
@@ -131,18 +130,23 @@ __device__ __forceinline__ int wave_argmax_abs(T v, bool live, unsigned long lon
 // workgroup also writes the factors of the chosen rows (lu11, column-major,
 // leading dimension CA_W), the LAPACK interchanges ipiv[k .. k + w), the move
 // list that performs them, and info at the first zero pivot.
-template <typename T, int G>
+template <typename T, int G, bool FINAL>
 __global__ void __launch_bounds__(G)
     k_calu_select(const T *__restrict__ A, int lda, int k, int w, const int *__restrict__ rows_in, int cnt,
-                  int *__restrict__ rows_out, int final_level, T *__restrict__ lu11, int *__restrict__ ipiv,
-                  int *__restrict__ moves, int *__restrict__ info)
+                  int *__restrict__ rows_out, T *__restrict__ lu11, int *__restrict__ ipiv, int *__restrict__ moves,
+                  int *__restrict__ info)
 {
     // per wave and column parity: the wave's best row (its key, thread, and
     // values), so that one barrier per column suffices
     __shared__ T wrow[2][G / 64][CA_W];
     __shared__ unsigned long long wk[2][G / 64];
     __shared__ int wt[2][G / 64];
-    __shared__ T s_u[CA_W][CA_W + 1];   // rows of U11 as they are chosen
+    // final level only: rows of U11 as they are chosen, every row's
+    // elimination multipliers (L11 is read from them), the thread of each
+    // chosen row
+    __shared__ T s_u[FINAL ? CA_W : 1][CA_W + 1];
+    __shared__ T mult[FINAL ? G : 1][CA_W + 1];
+    __shared__ int s_selt[CA_W];
     __shared__ int s_sel[CA_W];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int idx = blockIdx.x * G + t;
@@ -150,8 +154,8 @@ __global__ void __launch_bounds__(G)
     bool live = myrow >= 0;
     // The row stays in registers.  The elimination updates all 32 columns
     // unconditionally (columns <= c are never read again, so what they hold
-    // does not matter), and the multipliers are not kept: the final level
-    // recomputes L11 from the chosen rows.  Columns are visited in blocks of
+    // does not matter); the multipliers are kept (in LDS) at the final level
+    // only, where they form L11.  Columns are visited in blocks of
     // 8, unrolled inside a block, so column c is one of four registers.  (A
     // fully unrolled 32-column loop is ~20k instructions of straight-line
     // code, beyond the instruction cache.)
@@ -200,46 +204,39 @@ __global__ void __launch_bounds__(G)
             const T myv = wrow[par][bw][lane & (CA_W - 1)];
             const T piv = lane_value(myv, c);
             if (t == pt) {
-                if (final_level) {
-#pragma unroll
-                    for (int cc = 0; cc < CA_W; cc++) s_u[c][cc] = cc >= c ? a[cc] : (T)0;
-                }
                 s_sel[c] = myrow;
                 live = false;
             }
-            if (piv == (T)0 && final_level && t == 0 && *info == 0) *info = k + c + 1;
+            if (FINAL) {
+                if (wave == 0 && lane < CA_W) s_u[c][lane] = lane >= c ? myv : (T)0;
+                if (t == 0) {
+                    s_selt[c] = pt;
+                    if (piv == (T)0 && *info == 0) *info = k + c + 1;
+                }
+            }
             // branch-free: rows that are no longer live (and all rows at a
             // zero pivot) take a zero multiplier and keep their values
             T pr[CA_W];
 #pragma unroll
             for (int cc = 0; cc < CA_W; cc++) pr[cc] = lane_value(myv, cc);
             const T l = (live && piv != (T)0) ? ac / piv : (T)0;
+            if (FINAL && live) mult[t][c] = l;
 #pragma unroll
             for (int cc = 0; cc < CA_W; cc++) a[cc] = fma(-l, pr[cc], a[cc]);
         }
     }
     __syncthreads();
-    if (final_level && t < w) {
-        // row t of L11: x U11 = (chosen row t as it is in A), x_p for p < t
-        const int r = s_sel[t];
-        T x[CA_W];
-#pragma unroll
-        for (int c = 0; c < CA_W; c++) x[c] = (r >= 0 && c < w) ? A[r + (size_t)(k + c) * lda] : (T)0;
-#pragma unroll
-        for (int p = 0; p < CA_W; p++) {
-            if (p < t) {
-                x[p] = x[p] / s_u[p][p];
-#pragma unroll
-                for (int q = p + 1; q < CA_W; q++) x[q] = fma(-x[p], s_u[p][q], x[q]);
-            }
-        }
+    if (FINAL && t < w) {
+        // row t of L11 (the multipliers of the t-th chosen row against the
+        // pivots before it) and of U11
+        const int tt = s_selt[t];
 #pragma unroll
         for (int p = 0; p < CA_W; p++)
-            if (p < w) lu11[t + p * CA_W] = p < t ? x[p] : s_u[t][p];
+            if (p < w) lu11[t + p * CA_W] = p < t ? mult[tt][p] : s_u[t][p];
     }
     __syncthreads();
     if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
-    if (!final_level || wave != 0) return;
+    if (!FINAL || wave != 0) return;
     // LAPACK interchange sequence and the net row moves.  Lane e holds entry
     // e of the map "position key[e] now holds original row val[e]" over the
     // positions touched so far (at most 2 w <= 64).
@@ -247,9 +244,9 @@ __global__ void __launch_bounds__(G)
     for (int c = 0; c < w; c++) {
         const int s = s_sel[c];
         const unsigned long long bv = __ballot(lane < size && val == s);
-        const int p = bv ? __shfl(key, __ffsll((unsigned long long)bv) - 1, 64) : s;
+        const int p = bv ? __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)bv) - 1) : s;
         const unsigned long long bk = __ballot(lane < size && key == k + c);
-        const int cur = bk ? __shfl(val, __ffsll((unsigned long long)bk) - 1, 64) : k + c;
+        const int cur = bk ? __builtin_amdgcn_readlane(val, __ffsll((unsigned long long)bk) - 1) : k + c;
         if (lane == 0) ipiv[k + c] = p + 1;
         // position k + c receives s, position p receives cur
         if (bk) {
@@ -466,26 +463,27 @@ static int select_group(int cnt)
     return CA_G;
 }
 
+template <typename T, int G>
+static void launch_select_g(int groups, hipStream_t s, const T *A, int lda, int k, int w, const int *in, int cnt,
+                            int *out, int fin, T *lu11, int *ipiv, int *moves, int *info)
+{
+    if (fin)
+        hipLaunchKernelGGL((k_calu_select<T, G, true>), dim3(groups), dim3(G), 0, s, A, lda, k, w, in, cnt, out, lu11,
+                           ipiv, moves, info);
+    else
+        hipLaunchKernelGGL((k_calu_select<T, G, false>), dim3(groups), dim3(G), 0, s, A, lda, k, w, in, cnt, out,
+                           lu11, ipiv, moves, info);
+}
+
 template <typename T>
 static void launch_select(int g, int groups, hipStream_t s, const T *A, int lda, int k, int w, const int *in, int cnt,
                           int *out, int fin, T *lu11, int *ipiv, int *moves, int *info)
 {
     switch (g) {
-    case 64:
-        hipLaunchKernelGGL((k_calu_select<T, 64>), dim3(groups), dim3(64), 0, s, A, lda, k, w, in, cnt, out, fin, lu11,
-                           ipiv, moves, info);
-        break;
-    case 128:
-        hipLaunchKernelGGL((k_calu_select<T, 128>), dim3(groups), dim3(128), 0, s, A, lda, k, w, in, cnt, out, fin,
-                           lu11, ipiv, moves, info);
-        break;
-    case 256:
-        hipLaunchKernelGGL((k_calu_select<T, 256>), dim3(groups), dim3(256), 0, s, A, lda, k, w, in, cnt, out, fin,
-                           lu11, ipiv, moves, info);
-        break;
-    default:
-        hipLaunchKernelGGL((k_calu_select<T, CA_G>), dim3(groups), dim3(CA_G), 0, s, A, lda, k, w, in, cnt, out, fin,
-                           lu11, ipiv, moves, info);
+    case 64: launch_select_g<T, 64>(groups, s, A, lda, k, w, in, cnt, out, fin, lu11, ipiv, moves, info); break;
+    case 128: launch_select_g<T, 128>(groups, s, A, lda, k, w, in, cnt, out, fin, lu11, ipiv, moves, info); break;
+    case 256: launch_select_g<T, 256>(groups, s, A, lda, k, w, in, cnt, out, fin, lu11, ipiv, moves, info); break;
+    default: launch_select_g<T, CA_G>(groups, s, A, lda, k, w, in, cnt, out, fin, lu11, ipiv, moves, info);
     }
 }
 
