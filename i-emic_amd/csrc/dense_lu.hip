@@ -152,32 +152,34 @@ __global__ void __launch_bounds__(G)
     const int idx = blockIdx.x * G + t;
     const int myrow = idx < cnt ? (rows_in ? rows_in[idx] : k + idx) : -1;
     bool live = myrow >= 0;
-    // The row stays in registers.  The elimination updates all 32 columns
-    // unconditionally (columns <= c are never read again, so what they hold
-    // does not matter); the multipliers are kept (in LDS) at the final level
-    // only, where they form L11.  Columns are visited in blocks of
-    // 8, unrolled inside a block, so column c is one of four registers.  (A
-    // fully unrolled 32-column loop is ~20k instructions of straight-line
-    // code, beyond the instruction cache.)
+    // The row stays in registers.  The column loop is unrolled, so column c
+    // is a fixed register and each step touches only the columns after c:
+    // the pivot row's live part is broadcast through scalar registers and
+    // the rows updated branch-free (about 4k instructions in all; a step
+    // that updated every column, unrolled, was ~20k, beyond the instruction
+    // cache).  The multipliers are kept (in LDS) at the final level only,
+    // where they form L11.
     T a[CA_W];
 #pragma unroll
     for (int c = 0; c < CA_W; c++) a[c] = (live && c < w) ? A[myrow + (size_t)(k + c) * lda] : (T)0;
     if (t < CA_W) s_sel[t] = -1;
     __syncthreads();
     bool done = false;   // uniform: no rows left in this group
-#pragma unroll 1
-    for (int cb = 0; cb < w && !done; cb += 8) {
+#pragma unroll
+    for (int cb = 0; cb < CA_W; cb += 8) {
+        if (cb >= w || done) break;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int c = cb + j;
+            const int c = cb + j;   // a compile-time constant in each unrolled step
             if (c >= w || done) break;
-            const T ac = cb == 0 ? a[j] : cb == 8 ? a[8 + j] : cb == 16 ? a[16 + j] : a[24 + j];
+            const T ac = a[c];
             const int par = c & 1;
             unsigned long long key;
             const int fl = wave_argmax_abs(ac, live, &key);
             if (lane == fl) {
 #pragma unroll
-                for (int cc = 0; cc < CA_W; cc++) wrow[par][wave][cc] = a[cc];
+                for (int cc = 0; cc < CA_W; cc++)
+                    if (cc >= c) wrow[par][wave][cc] = a[cc];
             }
             if (lane == 0) {
                 wk[par][wave] = key;
@@ -208,7 +210,7 @@ __global__ void __launch_bounds__(G)
                 live = false;
             }
             if (FINAL) {
-                if (wave == 0 && lane < CA_W) s_u[c][lane] = lane >= c ? myv : (T)0;
+                if (wave == 0 && lane < CA_W) s_u[c][lane] = lane >= c ? myv : (T)0;   // myv: lanes >= c were written
                 if (t == 0) {
                     s_selt[c] = pt;
                     if (piv == (T)0 && *info == 0) *info = k + c + 1;
@@ -216,13 +218,11 @@ __global__ void __launch_bounds__(G)
             }
             // branch-free: rows that are no longer live (and all rows at a
             // zero pivot) take a zero multiplier and keep their values
-            T pr[CA_W];
-#pragma unroll
-            for (int cc = 0; cc < CA_W; cc++) pr[cc] = lane_value(myv, cc);
             const T l = (live && piv != (T)0) ? ac / piv : (T)0;
             if (FINAL && live) mult[t][c] = l;
 #pragma unroll
-            for (int cc = 0; cc < CA_W; cc++) a[cc] = fma(-l, pr[cc], a[cc]);
+            for (int cc = 0; cc < CA_W; cc++)
+                if (cc > c) a[cc] = fma(-l, lane_value(myv, cc), a[cc]);
         }
     }
     __syncthreads();
