@@ -1,0 +1,178 @@
+"""The benchmark workloads at their full sizes, held to the reference's own
+solve bar (Ocean::solve's true FP64 residual, Ocean.C:1129-1146) recomputed
+on the host from the exported Jacobian:
+
+* the 180x80x24 seeded global basin: one Newton step (F, J, preconditioner,
+  FGMRES to 1e-8), and the 32-column batch (24 forward columns J y, y ~
+  N(0,1), and 8 adjoint columns: the two steady-state diagnostics' gradients
+  and six J^T y);
+* the eight mixed-size seeded basins, one Newton step each, on concurrent
+  host threads and streams and one after the other;
+* the 96x38x12 seeded basin: pseudo-arclength continuation in COMB from
+  rest with the reference's continuation defaults (Continuation.H:1330-1385),
+  ten steps;
+* the preconditioner with FP32 factors (precision = 1) on the closed basins
+  where the FP32 row-block LU is accurate enough to converge: the FP64 bar,
+  and its apply against the FP64 factors' at an FP32-level tolerance.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import iemic_amd
+from gpu_helpers import gpu_ocean
+from thcm_cases import box16_case, load_golden
+
+pytestmark = pytest.mark.gpu
+
+EIGHT_BASINS = [(32, 32, 12), (48, 24, 12), (64, 32, 16), (96, 38, 12), (72, 36, 12), (40, 40, 8),
+                (80, 40, 16), (96, 48, 12)]
+
+
+def _global(n, m, l):
+    from iemic_amd import basins
+    cfg_d, lm, frc, x0, pars = basins.make_global(n, m, l)
+    cfg = iemic_amd.Config(**{f: cfg_d[f] for f, _ in iemic_amd.Config._fields_})
+    oc = iemic_amd.Ocean(cfg, lm, frc["taux"], frc["tauy"], frc["tatm"], frc["emip"], frc["spert"])
+    for k, v in pars.items():
+        oc.setPar(k, v)
+    return oc, x0
+
+
+def _csr(oc):
+    rp, ci, val, _ = oc.getJacobian()
+    return sp.csr_matrix((val, ci, rp), shape=(oc.ndim, oc.ndim))
+
+
+@pytest.fixture(scope="module")
+def basin180():
+    print("building the 180x80x24 seeded basin", flush=True)
+    oc, x0 = _global(180, 80, 24)
+    yield oc, x0
+    oc.close()
+
+
+def test_newton_step_at_180x80x24(basin180):
+    oc, x0 = basin180
+    F = oc.computeRHS(x0)
+    oc.computeJacobian(x0)
+    y, info = oc.solve(F, oc.solver_params(tol=1e-8, restart=50, maxit=200))
+    J = _csr(oc)
+    rel = np.linalg.norm(F - J @ y) / np.linalg.norm(F)
+    print(f"180x80x24 Newton step: {info.iterations} FGMRES iterations, true residual {rel:.2e} "
+          f"(reported {info.rel_residual:.2e}), build {info.t_precon_setup:.2f} s, solve {info.t_solve:.3f} s")
+    assert info.converged and rel <= 1e-8
+    assert abs(rel - info.rel_residual) <= 1e-2 * rel + 1e-14
+
+
+def test_batch_of_32_at_180x80x24(basin180):
+    """24 forward and 8 adjoint columns, each to the FP64 bar."""
+    oc, x0 = basin180
+    oc.computeJacobian(x0)
+    J = _csr(oc)
+    rng = np.random.default_rng(13)
+    p = oc.solver_params(tol=1e-8, restart=10, maxit=100)
+    B = np.stack([J @ rng.normal(size=oc.ndim) for _ in range(24)], axis=1)
+    X, infos = oc.solveMulti(B, p)
+    rels = [np.linalg.norm(B[:, q] - J @ X[:, q]) / np.linalg.norm(B[:, q]) for q in range(24)]
+    print(f"24 forward columns: iterations {[i.iterations for i in infos]}, max residual {max(rels):.2e}")
+    assert max(rels) <= 1e-8
+    del B, X
+    G = [oc.diagnostic(iemic_amd.DIAG_MEAN_SST, x0)[1], oc.diagnostic(iemic_amd.DIAG_MAX_PSIM, x0)[1]]
+    G += [J.T @ rng.normal(size=oc.ndim) for _ in range(6)]
+    G = np.stack(G, axis=1)
+    L, infos = oc.solveMulti(G, p, transpose=True)
+    rels = [np.linalg.norm(G[:, q] - J.T @ L[:, q]) / np.linalg.norm(G[:, q]) for q in range(8)]
+    print(f"8 adjoint columns: iterations {[i.iterations for i in infos]}, max residual {max(rels):.2e}")
+    assert max(rels) <= 1e-8
+
+
+def test_eight_mixed_basins_concurrently_and_one_by_one():
+    models = []
+    for s in EIGHT_BASINS:
+        print(f"building basin {s}", flush=True)
+        models.append(_global(*s))
+    oceans = [oc for oc, _ in models]
+    dxs = [iemic_amd.DeviceArray(oc.ndim).upload(x0) for oc, x0 in models]
+    p = oceans[0].solver_params(tol=1e-8, restart=50, maxit=200)
+    out = {}
+    for conc in (False, True):
+        print("concurrent" if conc else "sequential", flush=True)
+        ddx = [iemic_amd.DeviceArray(oc.ndim) for oc in oceans]
+        infos = iemic_amd.newton_step_many(oceans, dxs, ddx, p, concurrent=conc)
+        out[conc] = [d.download() for d in ddx]
+        print([(i.iterations, f"{i.rel_residual:.1e}") for i in infos])
+        assert all(i.converged for i in infos)
+    for q, (oc, x0) in enumerate(models):
+        a, b = out[False][q], out[True][q]
+        F = oc.computeRHS(x0)
+        oc.computeJacobian(x0)
+        J = _csr(oc)
+        rel = np.linalg.norm(F - J @ b) / np.linalg.norm(F)
+        d = np.abs(a - b).max() / np.abs(a).max()
+        print(f"basin {EIGHT_BASINS[q]}: residual {rel:.2e}, concurrent vs sequential {d:.2e}")
+        assert rel <= 1e-8
+        assert d <= 1e-8
+    for oc in oceans:
+        oc.close()
+
+
+def test_continuation_on_the_96x38x12_basin():
+    """Ten pseudo-arclength steps in COMB from rest, the reference's
+    defaults (corrector tolerance 1e-4, ds 1e-2) and up to 8 Jacobians per
+    factorisation: every step's corrector converges, COMB increases, and
+    the end point is a steady state (Newton at the reached COMB)."""
+    oc, _ = _global(96, 38, 12)
+    oc.setPar("COMB", 0.0)
+    oc.setPreconditionerReuse(8)
+    cp = oc.continuation_params(par=19, target=1.0, max_steps=10)
+    x, ph, nh, info = oc.continuation(np.zeros(oc.ndim), cp)
+    print(f"steps {info.steps}, rejected {info.rejected}, Newton {info.newton_iterations}, "
+          f"linear {info.linear_iterations}, {info.seconds:.1f} s")
+    print("COMB", np.array2string(ph, precision=4))
+    assert info.steps == 10
+    assert np.all(np.diff(ph) > 0)
+    oc.setPar("COMB", float(ph[-1]))
+    F = oc.computeRHS(x)
+    print(f"|F| at the last point {np.linalg.norm(F):.2e} (COMB {ph[-1]:.4f}, |x| {nh[-1]:.3e})")
+    assert np.linalg.norm(F) <= 1e-3
+    oc.close()
+
+
+def _small(name):
+    if name == "box16":
+        return box16_case()
+    case, _ = load_golden(name)
+    return case
+
+
+@pytest.mark.parametrize("name", ["natl8", "box16"])
+def test_fp32_factors_reach_the_fp64_bar(name):
+    """precision = 1: the row-block LU factored and applied in FP32 inside
+    the FP64 FGMRES.  The chain of Schur complements D_j = A_jj - A_j,j-1
+    D_j-1^{-1} A_j-1,j pivots only inside each row block, so its FP32 rounding
+    is amplified by the conditioning of the D_j: on natl8 the FP32 apply
+    differs from the FP64 one by about 5e-3 of max |M^{-1} v| (FP32 unit
+    roundoff 6e-8).  That is still a preconditioner FGMRES corrects in a few
+    more iterations to the FP64 bar on these closed basins; on the periodic
+    global basins the amplification grows past what FGMRES recovers."""
+    case = _small(name)
+    oc = gpu_ocean(case)
+    F = oc.computeRHS(case.x)
+    oc.computeJacobian(case.x)
+    J = _csr(oc)
+    v = np.random.default_rng(5).normal(size=oc.ndim)
+    z64 = oc.applyPrecon(v)
+    z32 = oc.applyPreconFP32(v)
+    err = np.abs(z32 - z64).max() / np.abs(z64).max()
+    p64 = oc.solver_params(tol=1e-10, restart=50, maxit=200)
+    p32 = oc.solver_params(tol=1e-10, restart=50, maxit=200, precision=1)
+    _, i64 = oc.solve(F, p64)
+    y, i32 = oc.solve(F, p32)
+    rel = np.linalg.norm(F - J @ y) / np.linalg.norm(F)
+    print(f"{name}: FP32 apply vs FP64 {err:.2e}; iterations FP64 {i64.iterations}, FP32 factors "
+          f"{i32.iterations}; true residual {rel:.2e}")
+    assert err <= 5e-2
+    assert rel <= 1e-8
+    assert i32.iterations <= 3 * i64.iterations + 10
+    oc.close()
