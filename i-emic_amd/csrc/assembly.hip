@@ -1073,6 +1073,27 @@ int launch_rhs(iemic_ctx *c, const double *d_x, double *d_F)
     return IEMIC_OK;
 }
 
+// F(x) with the integral row's reduction partials in a caller-provided
+// device buffer of cap doubles (one partial per 256 unknowns), no timing
+// events and no allocation: the kernels of launch_rhs, one partial per block.
+int launch_rhs_partials(iemic_ctx *c, const double *d_x, double *d_F, double *d_part, int cap)
+{
+    const int nb = nblk(c->sc.ndim, 256);
+    if (c->sc.rowintcon >= 0 && nb > cap)
+        return set_error(IEMIC_EINVAL, "reduction needs " + std::to_string(nb) + " partials, the buffer holds " +
+                                           std::to_string(cap));
+    KArgs g = kargs(c);
+    hipLaunchKernelGGL(k_rhs, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, g, d_x, d_F);
+    HIPCHK(hipGetLastError());
+    if (c->sc.rowintcon >= 0) {
+        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(256), 0, c->stream, c->d_coef, d_x, c->sc.ndim, d_part);
+        hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, c->stream, d_part, nb, d_F, (double)c->cfg.int_sign,
+                           c->intcorr, c->sc.rowintcon);
+        HIPCHK(hipGetLastError());
+    }
+    return IEMIC_OK;
+}
+
 int launch_jacobian(iemic_ctx *c, const double *d_x)
 {
     if (!c->d_J) HIPCHK(hipMalloc(&c->d_J, (size_t)NSLOT * c->sc.ncell * sizeof(double)));

@@ -84,6 +84,7 @@ int set_error(int code, const std::string &msg);
 // assembly.hip
 int upload_tables(iemic_ctx *c);
 int launch_rhs(iemic_ctx *c, const double *d_x, double *d_F);
+int launch_rhs_partials(iemic_ctx *c, const double *d_x, double *d_F, double *d_part, int cap);
 int launch_jacobian(iemic_ctx *c, const double *d_x);
 int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y);

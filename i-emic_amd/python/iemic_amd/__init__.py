@@ -106,6 +106,7 @@ def load():
         "iemic_set_intcond_correction_value": ([vp, C.c_double], C.c_int),
         "iemic_rhs": ([vp, dp, dp], C.c_int),
         "iemic_rhs_dev": ([vp, vp, vp], C.c_int),
+        "iemic_rhs_dev_noalloc": ([vp, vp, vp], C.c_int),
         "iemic_jacobian": ([vp, dp], C.c_int),
         "iemic_jacobian_dev": ([vp, vp], C.c_int),
         "iemic_jacobian_csr": ([vp, ip, ip, dp, dp], C.c_int),
@@ -468,6 +469,10 @@ class Ocean:
     # ---- device-resident variants (DeviceArray arguments)
     def computeRHS_dev(self, dx, dF):
         _check(self.lib.iemic_rhs_dev(self.h, dx.ptr, dF.ptr))
+
+    def computeRHS_dev_noalloc(self, dx, dF):
+        """F(x) into dF with no device allocation (iemic_rhs_dev_noalloc)."""
+        _check(self.lib.iemic_rhs_dev_noalloc(self.h, dx.ptr, dF.ptr))
 
     def computeJacobian_dev(self, dx):
         _check(self.lib.iemic_jacobian_dev(self.h, dx.ptr))

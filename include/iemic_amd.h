@@ -123,6 +123,20 @@ int iemic_set_intcond_correction_value(iemic_ctx *ctx, double value);
 int iemic_rhs(iemic_ctx *ctx, const double *x, double *F);
 int iemic_rhs_dev(iemic_ctx *ctx, const double *d_x, double *d_F);
 
+/* F(x) as iemic_rhs_dev, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * integral row's reduction partials (one per 256 unknowns) live in a
+ * library-owned device array of IEMIC_NOALLOC_MAX_NDIM / 256 doubles
+ * (128 KiB on each GPU the library runs on, held until the process ends)
+ * instead of a buffer of the context; ndim above IEMIC_NOALLOC_MAX_NDIM is
+ * rejected with IEMIC_EINVAL.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other contexts, streams or threads) race.
+ * F equals iemic_rhs_dev's in every row; the integral row sums its terms in
+ * another order.  The context itself is allocated by iemic_create. */
+#define IEMIC_NOALLOC_MAX_NDIM 4194304
+int iemic_rhs_dev_noalloc(iemic_ctx *ctx, const double *d_x, double *d_F);
+
 /* ---- Ocean::computeJacobian (Ocean.C:1297-1309); J stays on the device -- */
 int iemic_jacobian(iemic_ctx *ctx, const double *x);
 int iemic_jacobian_dev(iemic_ctx *ctx, const double *d_x);
