@@ -559,8 +559,11 @@ namespace iemic {
 //  0  area-mean SST: sum over ocean surface cells of cos(y_j) T / sum cos(y_j)
 //     (T is the anomaly unknown; a constant offset does not change gradients)
 //  1  maximum of the meridional overturning streamfunction, the quantity
-//     OceanGrid::psimMax reports (OceanGrid.C:270-345, compute_psim in
-//     thcm_utils.F90:95-118): vs(j,k) = dx sum_i v(i,j,k),
+//     OceanGrid::psimMax reports (OceanGrid.C:264-345, compute_psim in
+//     thcm_utils.F90:95-118): vs(j,k) = dx sum_i v(i,j,k) over the v field
+//     that usol (usrc.F90:1014-1121) unpacks, which is zero on the north
+//     wall (j = m), on the east wall of a closed basin (i = n) and at every
+//     velocity point next to a land cell;
 //     psi(j,k) = -cos(yv_j) vs(j,k) dz dfzT(k) + psi(j,k-1) below 500 m;
 //     the gradient is that of psi at its (first) maximiser.
 int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad)
@@ -589,6 +592,15 @@ int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, do
         return IEMIC_OK;
     }
     if (which == 1) {
+        // v(i, j, k) as usol leaves it: the cells (i, j), (i + 1, j),
+        // (i, j + 1), (i + 1, j + 1) inside 1..n x 1..m that are land zero it
+        auto vlive = [&](int i, int j, int k) {
+            if (j >= m) return false;
+            if (i >= n && !c->cfg.periodic) return false;
+            if (lm(c, i, j, k) == LAND || lm(c, i, j + 1, k) == LAND) return false;
+            if (i < n && (lm(c, i + 1, j, k) == LAND || lm(c, i + 1, j + 1, k) == LAND)) return false;
+            return true;
+        };
         double best = 0.0;
         int bj = -1, bk = -1;
         std::vector<double> psi((size_t)(m + 1) * (l + 1), 0.0);
@@ -597,7 +609,7 @@ int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, do
             for (int k = 1; k <= l; k++) {
                 if (!(h.z[k] * c->cfg.hdim < -500.0)) continue;
                 double vs = 0.0;
-                for (int i = 1; i <= n; i++) vs += x[row(i, j, k, V)];
+                for (int i = 1; i <= n; i++) vs += vlive(i, j, k) ? x[row(i, j, k, V)] : 0.0;
                 vs *= h.dx;
                 psi[j + (size_t)(m + 1) * k] = -cs * vs * h.dz * h.dfzT[k] + psi[j + (size_t)(m + 1) * (k - 1)];
             }
@@ -615,7 +627,8 @@ int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, do
             for (int k = 1; k <= bk; k++) {
                 if (!(h.z[k] * c->cfg.hdim < -500.0)) continue;
                 const double w = -std::cos(h.yv[bj]) * h.dx * h.dz * h.dfzT[k];
-                for (int i = 1; i <= n; i++) grad[row(i, bj, k, V)] = w;
+                for (int i = 1; i <= n; i++)
+                    if (vlive(i, bj, k)) grad[row(i, bj, k, V)] = w;
             }
         return IEMIC_OK;
     }

@@ -380,6 +380,46 @@ int thcmref_rhs(const double *x, double intcorr, double *F)
     return 0;
 }
 
+/* OceanGrid::recomputePsiM on one rank (OceanGrid.C:264-345): the state
+ * unpacked onto the grid by the reference's usol1D (thcm_utils.F90:190-254,
+ * usol), v summed over i = 1..n times dx = (xmax - xmin) / n for
+ * j = 0..m, k = 1..l, then the reference's compute_psim
+ * (thcm_utils.F90:95-118).  psim receives psiM(0:m, 0:l), j fastest; the
+ * return value is the maximum over it, started at psiM(0,0) as the
+ * reference's scan is. */
+extern void _QMm_thcm_utilsPusol1d(double *un, double *u, double *v, double *w, double *p, double *t,
+                                   double *s);
+extern void _QMm_thcm_utilsPcompute_psim(double *vs, double *psim);
+double thcmref_psim(const double *x, double *psim)
+{
+    const double PI_ = 3.14159265358979323846;
+    int n = g_cfg.n, m = g_cfg.m, l = g_cfg.l;
+    size_t nuv = (size_t)(n + 1) * (m + 1) * (l + 2), nw = (size_t)(n + 2) * (m + 2) * (l + 1);
+    size_t np = (size_t)(n + 2) * (m + 2) * (l + 2);
+    double *xx = malloc(sizeof(double) * g_ndim);
+    double *u = calloc(nuv, sizeof(double)), *v = calloc(nuv, sizeof(double));
+    double *w = calloc(nw, sizeof(double)), *p = calloc(np, sizeof(double));
+    double *t = calloc(np, sizeof(double)), *sa = calloc(np, sizeof(double));
+    double *vs = calloc((size_t)(m + 1) * l, sizeof(double));
+    memcpy(xx, x, sizeof(double) * g_ndim);
+    _QMm_thcm_utilsPusol1d(xx, u, v, w, p, t, sa);
+    double xmin = g_cfg.xmin_deg * PI_ / 180.0, xmax = g_cfg.xmax_deg * PI_ / 180.0;
+    double dx = (xmax - xmin) / n;
+    for (int k = 1; k <= l; k++)
+        for (int j = 0; j <= m; j++) {
+            double sum = 0.0;
+            for (int i = 1; i <= n; i++) sum += v[((size_t)(m + 1) * k + j) * (n + 1) + i];
+            vs[(size_t)(m + 1) * (k - 1) + j] = sum * dx;
+        }
+    _QMm_thcm_utilsPcompute_psim(vs, psim);
+    double mx = psim[0];
+    for (int j = 0; j <= m; j++)
+        for (int k = 0; k <= l; k++)
+            if (psim[(size_t)(m + 1) * k + j] > mx) mx = psim[(size_t)(m + 1) * k + j];
+    free(xx); free(u); free(v); free(w); free(p); free(t); free(sa); free(vs);
+    return mx;
+}
+
 /* The Fortran CSR of the last rhs_/matrix_ call (1-based), for inspection. */
 int thcmref_fortran_csr_nnz(void) { return g_begA[g_ndim] - 1; }
 void thcmref_fortran_csr(int *beg, int *jco, double *co)

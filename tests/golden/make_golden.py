@@ -13,7 +13,10 @@ F(x) as Ocean::computeRHS gives it, the Jacobian on the fixed graph (row
 pointer, columns, values; all rows, or a seeded row sample for the larger
 grids), diag(B), the integral-condition coefficients and row.  Land masks of
 the natl8 / gateway / global cases are the reference's own data files
-(data/mkmask), read by the reference's readmask.
+(data/mkmask), read by the reference's readmask.  Each fixture also holds the
+meridional overturning streamfunction psiM(0:m, 0:l) of its state and its
+maximum, as OceanGrid::recomputePsiM forms them with the reference's usol1D
+and compute_psim (thcm_utils.F90:95-118, 190-254).
 """
 import json
 import os
@@ -41,6 +44,8 @@ def write(case, row_frac=None):
     out = dict(cfg=json.dumps(config_dict(case.cfg)), pars=json.dumps(case.pars),
                landm=case.landm, x=case.x, intcorr=np.float64(case.intcorr),
                F=F, diagB=dB, coef=ref.coef(), rowintcon=np.int32(ref.rowintcon()))
+    # OceanGrid::recomputePsiM through the reference's usol1D and compute_psim
+    out["psim_max"], out["psim"] = ref.psim()
     for k, v in case.fields.items():
         if v is not None:
             out["fld_" + k] = v

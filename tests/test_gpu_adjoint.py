@@ -219,3 +219,26 @@ def test_torch_autograd_wrapper_gives_the_adjoint_gradient(natl8_steady, fd_tabl
         _, _, fd, h = fd_table[which, par]
         print(f"{par}: autograd {p.grad[i].item():.12e} fd {fd:.12e}")
         assert abs(p.grad[i].item() - fd) <= 1e-6 * abs(fd) + 1e-12 * abs(g.item()) / h
+
+
+@pytest.mark.parametrize("name", ["box16", "natl8", "natl8_eos", "gateway16", "global96", "reft16"])
+def test_max_psim_matches_the_reference(name):
+    """g1 = max psiM equals, bit for bit, what the reference's own pipeline
+    (usol1D, the sum over i and compute_psim; OceanGrid::recomputePsiM,
+    OceanGrid.C:264-345) gives on each fixture's state, stored in the fixture
+    by tests/golden/make_golden.py.  Its gradient is exact: psiM is linear in
+    v, so a step along the gradient's support that keeps the maximiser moves
+    g1 by grad . d."""
+    case, exp = load_golden(name)
+    oc = gpu_ocean(case)
+    g, grad = oc.diagnostic(iemic_amd.DIAG_MAX_PSIM, case.x)
+    print(f"{name}: max psiM {g!r} (reference {float(exp['psim_max'])!r})")
+    assert g == float(exp["psim_max"])
+    sup = np.nonzero(grad)[0]
+    assert len(sup) > 0 and np.all(sup % 6 == 1)
+    d = np.zeros(oc.ndim)
+    d[sup] = np.random.default_rng(3).normal(size=len(sup))
+    h = 1e-6 * max(abs(g), 1e-12) / np.abs(grad @ d)
+    g2, _ = oc.diagnostic(iemic_amd.DIAG_MAX_PSIM, case.x + h * d)
+    assert abs((g2 - g) - h * (grad @ d)) <= 1e-9 * abs(h * (grad @ d))
+    oc.close()

@@ -101,6 +101,8 @@ def load_ref():
         lib.thcmref_jacobian.argtypes = [_P(C.c_double), _P(C.c_int), _P(C.c_int),
                                          _P(C.c_double), _P(C.c_double)]
         lib.thcmref_fortran_csr.argtypes = [_P(C.c_int), _P(C.c_int), _P(C.c_double)]
+        lib.thcmref_psim.argtypes = [_P(C.c_double), _P(C.c_double)]
+        lib.thcmref_psim.restype = C.c_double
         _ref_lib = lib
     return _ref_lib
 
@@ -264,6 +266,14 @@ class RefModel:
         dropped = on_big_stack(self.lib.thcmref_jacobian, _ptr(x, C.c_double), _ptr(rp, C.c_int),
                                _ptr(ci, C.c_int), _ptr(val, C.c_double), _ptr(dB, C.c_double))
         return rp, ci, val, dB, dropped
+
+    def psim(self, x=None):
+        """(max psiM, psiM(0:m, 0:l) j fastest) of OceanGrid::recomputePsiM."""
+        x = self.case.x if x is None else np.ascontiguousarray(x, np.float64)
+        c = self.case.cfg
+        out = np.zeros((c.m + 1) * (c.l + 1))
+        mx = on_big_stack(self.lib.thcmref_psim, _ptr(x, C.c_double), _ptr(out, C.c_double))
+        return mx, out
 
     def fortran_csr(self):
         nz = self.lib.thcmref_fortran_csr_nnz()
