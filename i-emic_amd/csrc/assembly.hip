@@ -738,20 +738,21 @@ __global__ void __launch_bounds__(256) k_jacobian(KArgs g, const double *__restr
     bnd_p(b, a); store_row<53, 11>(J, C, cell, a);
     FdMix fd;
     double bT = 0.0, bS = 0.0;
-    if (g.s.vmix_on) mixing_block(x, g, i, j, k, fd, bT, bS);
+    const bool mixj = g.s.vmix_on && (g.s.mix_t || g.s.mix_sj);
+    if (mixj) mixing_block(x, g, i, j, k, fd, bT, bS);
     // T row: mixing entries at (14|5|23, T|S) = slots 16|11|18 and 17|12|19
     const int sl[3][2] = {{16, 17}, {11, 12}, {18, 19}};
     row_ts<true, T>(g, q, al + ROW0[T], j, k, a);
-    if (g.s.vmix_on)
+    if (mixj && g.s.mix_t)
 #pragma unroll
         for (int d = 0; d < 3; d++)
             if (fd.lev[d]) {
                 a[sl[d][0]] = a[sl[d][0]] + fd.T[d][0];
-                a[sl[d][1]] = a[sl[d][1]] + fd.T[d][1];
+                if (g.s.mix_ts) a[sl[d][1]] = a[sl[d][1]] + fd.T[d][1];
             }
     bnd_ts(b, a, 11); store_row<64, 20>(J, C, cell, a);
     row_ts<true, S>(g, q, al + ROW0[S], j, k, a);
-    if (g.s.vmix_on)
+    if (mixj && g.s.mix_sj)
 #pragma unroll
         for (int d = 0; d < 3; d++)
             if (fd.lev[d]) {
@@ -785,11 +786,13 @@ __global__ void __launch_bounds__(256) k_rhs(KArgs g, const double *__restrict__
     const double *al = g.al + ((size_t)(k - 1) * m + (j - 1)) * NSLOT;
     const double *frc = g.frc + 6 * (size_t)cell;
     double mT = 0.0, mS = 0.0;
-    if (g.s.vmix_on) {
+    if (g.s.vmix_on && (g.s.mix_t || g.s.mix_sf)) {
         MixCol mc;
         mix_col(x, g, i, j, k, mc);
         Face f0 = face_of(g, mc, 0, mc.t, mc.s), f1 = face_of(g, mc, 1, mc.t, mc.s);
         mix_div(g, k, f0, f1, mT, mS);
+        if (!g.s.mix_t) mT = 0.0;   // vmix_fun's rows are 0.0 while switched off
+        if (!g.s.mix_sf) mS = 0.0;
     }
     double a[24], au;
     // B = -Au - mix + Frc (usrc.F90:576), F = -B (THCM.C:1011)
@@ -1060,8 +1063,71 @@ static int dot_into(iemic_ctx *c, const double *a, const double *b, int n, doubl
     return IEMIC_OK;
 }
 
+// Sums of squares of the T and S unknowns (l2nrm of vmix_control,
+// matetc.F90:22-36), one pair of partials per block.
+__global__ void __launch_bounds__(256) k_ts_sumsq(const double *__restrict__ x, int ncell, double *__restrict__ part)
+{
+    __shared__ double st[256], ss[256];
+    double at = 0.0, as = 0.0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        const double t = x[6 * (size_t)c + T], s = x[6 * (size_t)c + S];
+        at += t * t;
+        as += s * s;
+    }
+    st[threadIdx.x] = at;
+    ss[threadIdx.x] = as;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            st[threadIdx.x] += st[threadIdx.x + w];
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = st[0];
+        part[2 * blockIdx.x + 1] = ss[0];
+    }
+}
+
+// Mixing = 2: the control that Ocean::computeRHS / computeJacobian run before
+// every evaluation (THCM::fixMixing(0), then vmix_control, mix_imp.f:131-166):
+// the T / S switches follow whether the T / S fields are nonzero (2-norm
+// above 1e-12); when they change and T is on, the pattern is rebuilt with the
+// new switches (vmix_part with vmix_el_2, mix_imp.f:168-227, 934-1049).
+// d_part holds cap doubles of device scratch; one host synchronisation.
+int mixing_control(iemic_ctx *c, const double *d_x, double *d_part, int cap)
+{
+    if (c->vmix_flag != 2) return IEMIC_OK;
+    constexpr int MAXB = 512;
+    const int nb = std::min(std::min(MAXB, cap / 2), nblk(c->sc.ncell, 256));
+    if (nb < 1) return set_error(IEMIC_EINVAL, "mixing control: no scratch for the field norms");
+    double h[2 * MAXB];
+    hipLaunchKernelGGL(k_ts_sumsq, dim3(nb), dim3(256), 0, c->stream, d_x, c->sc.ncell, d_part);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h, d_part, 2 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double st = 0.0, ss = 0.0;
+    for (int b = 0; b < nb; b++) {
+        st += h[2 * b];
+        ss += h[2 * b + 1];
+    }
+    const int tt = std::sqrt(st) > 1.0e-12 ? 1 : 0, ts = std::sqrt(ss) > 1.0e-12 ? 1 : 0;
+    if (tt != c->mix2_temp || ts != c->mix2_salt) {
+        c->mix2_temp = tt;
+        c->mix2_salt = ts;
+        if (tt != 0) {
+            c->mix2_part = 1;
+            c->mix2_pat_salt = ts;
+        }
+    }
+    set_mix_switches(c);
+    return IEMIC_OK;
+}
+
 int launch_rhs(iemic_ctx *c, const double *d_x, double *d_F)
 {
+    if (int rc = mixing_control(c, d_x, c->d_red, c->nred)) return rc;
     KArgs g = kargs(c);
     timed_begin(c, 0);
     hipLaunchKernelGGL(k_rhs, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, g, d_x, d_F);
@@ -1078,6 +1144,7 @@ int launch_rhs(iemic_ctx *c, const double *d_x, double *d_F)
 // events and no allocation: the kernels of launch_rhs, one partial per block.
 int launch_rhs_partials(iemic_ctx *c, const double *d_x, double *d_F, double *d_part, int cap)
 {
+    if (int rc = mixing_control(c, d_x, d_part, cap)) return rc;
     const int nb = nblk(c->sc.ndim, 256);
     if (c->sc.rowintcon >= 0 && nb > cap)
         return set_error(IEMIC_EINVAL, "reduction needs " + std::to_string(nb) + " partials, the buffer holds " +
@@ -1096,6 +1163,15 @@ int launch_rhs_partials(iemic_ctx *c, const double *d_x, double *d_F, double *d_
 
 int launch_jacobian(iemic_ctx *c, const double *d_x)
 {
+    if (int rc = mixing_control(c, d_x, c->d_red, c->nred)) return rc;
+    if (c->sc.mix_t && !c->sc.mix_ts)
+        // the reference differences the T columns in the column groups that
+        // its DSM colouring builds for this pattern, and the salinity columns
+        // it leaves out of the pattern are perturbed with those groups: the
+        // entries then depend on that colouring, which is not restated here
+        return set_error(IEMIC_EUNSUPPORTED,
+                         "Mixing = 2 Jacobian at a state whose mixing pattern was built with a zero salinity field "
+                         "(its entries depend on the reference's DSM column grouping)");
     if (!c->d_J) HIPCHK(hipMalloc(&c->d_J, (size_t)NSLOT * c->sc.ncell * sizeof(double)));
     KArgs g = kargs(c);
     timed_begin(c, 1);

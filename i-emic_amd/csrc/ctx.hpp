@@ -47,6 +47,9 @@ struct iemic_ctx {
     bool graph_built = false;
     double intcorr = 0.0;
     int vmix_flag = 1;
+    // Mixing = 2 (mix_imp.f:131-227): current T / S switches, and the pattern
+    // of the finite-difference mixing Jacobian (built at all; with salinity)
+    int mix2_temp = 0, mix2_salt = 0, mix2_part = 0, mix2_pat_salt = 0;
 
     hipStream_t stream = nullptr;
     // device data
@@ -78,6 +81,7 @@ int host_init(iemic_ctx *c, const int *landm, const double *taux, const double *
               const double *tatm, const double *emip, const double *spert);
 void host_parameters_changed(iemic_ctx *c);   // forcing + linear tables (setparcs_)
 void build_graph(iemic_ctx *c);
+void set_mix_switches(iemic_ctx *c);   // mixing switches of the kernels (Mixing = 1 or 2)
 int diagnostic(const iemic_ctx *c, int which, const double *x, double *value, double *grad);
 int read_mask_text(const char *path, int n, int m, int l, int *landm);
 int set_error(int code, const std::string &msg);
@@ -85,6 +89,7 @@ int set_error(int code, const std::string &msg);
 int upload_tables(iemic_ctx *c);
 int launch_rhs(iemic_ctx *c, const double *d_x, double *d_F);
 int launch_rhs_partials(iemic_ctx *c, const double *d_x, double *d_F, double *d_part, int cap);
+int mixing_control(iemic_ctx *c, const double *d_x, double *d_part, int cap);
 int launch_jacobian(iemic_ctx *c, const double *d_x);
 int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y);

@@ -412,6 +412,7 @@ void build_scalars(iemic_ctx *c)
     s.fac = c->cfg.alphaT * par[SPL1];
     s.rho_mixing = c->cfg.rho_mixing ? 1 : 0;
     s.vmix_on = (c->vmix_flag >= 1) ? 1 : 0;
+    set_mix_switches(c);
 }
 
 // assemble.F90:18-54 fillcolB and THCM.C:1157-1160, 2197-2198
@@ -434,6 +435,26 @@ void build_diagB(iemic_ctx *c)
 
 }  // namespace
 
+// The mixing switches of the kernels.  Mixing = 1: every term.  Mixing = 2:
+// vmix_fun adds the T (S) rows' mixing only while the T (S) field is
+// nonzero and once a pattern exists (mix_imp.f:505-556, usrc.F90:551-563);
+// vmix_jac differences only the pattern's entries (vmix_el_2,
+// mix_imp.f:934-1049): T rows against T, and against S and the S rows when
+// the pattern was built with a nonzero S field.
+void set_mix_switches(iemic_ctx *c)
+{
+    Scalars &s = c->sc;
+    if (c->vmix_flag == 2) {
+        s.mix_t = c->mix2_part && c->mix2_temp;
+        s.mix_ts = s.mix_t && c->mix2_pat_salt;
+        s.mix_sf = c->mix2_part && c->mix2_salt;
+        s.mix_sj = s.mix_sf && c->mix2_pat_salt;
+    } else {
+        s.mix_t = s.mix_ts = s.mix_sf = s.mix_sj = 1;
+    }
+}
+
+
 void host_parameters_changed(iemic_ctx *c)
 {
     forcing(c);
@@ -450,8 +471,8 @@ int host_init(iemic_ctx *c, const int *landm, const double *taux, const double *
     if (n < 3 || m < 2 || l < 2) return set_error(IEMIC_EINVAL, "grid must be at least 3x2x2");
     if (g.coupled_t || g.coupled_s)
         return set_error(IEMIC_EUNSUPPORTED, "coupled ocean (Coupled Temperature/Salinity) is not supported");
-    if (g.vmix != 0 && g.vmix != 1)
-        return set_error(IEMIC_EUNSUPPORTED, "Mixing must be 0 or 1");
+    if (g.vmix != 0 && g.vmix != 1 && g.vmix != 2)
+        return set_error(IEMIC_EUNSUPPORTED, "Mixing must be 0, 1 or 2");
     if (g.int_sign != 1 && g.int_sign != -1) return set_error(IEMIC_EINVAL, "Salinity Integral Sign must be +-1");
     Host &h = c->h;
     h.xmin = g.xmin_deg * kPi / 180.0; h.xmax = g.xmax_deg * kPi / 180.0;

@@ -82,6 +82,12 @@ struct Scalars {
     int ncell, ndim;
     int rowintcon;        // -1 if no integral condition
     int vmix_on;          // vmix_flag >= 1 with T and S active
+    // switches of the mixing terms (all 1 under Mixing = 1; under Mixing = 2
+    // set before every F and J by the state-dependent control, mix_imp.f:131-227):
+    int mix_t;            //   T rows: mixing in F, and J's entries against T
+    int mix_ts;           //   T rows: J's entries against S
+    int mix_sf;           //   S rows: mixing in F
+    int mix_sj;           //   S rows: J's entries against T and S
     int rho_mixing;
     double epsr, Ra, xes, lambda;
     double alpt2, alpt3;

@@ -56,6 +56,8 @@ extern void _QMm_matPget_array_sizes(int *nrows, int *nnz);
 extern void _QMm_matPset_pointers(int *nrows, int *nnz, int *beg, int *jco, double *co,
                                   double *cob, int *begf, int *jcof, double *cof);
 extern void _QMm_thcm_utilsPintcond_scaling(double *val, int *ind, int *len);
+extern void _QMm_mixPset_vmix_fix(int *vmix_fix);
+extern int _QMm_mixEvmix_dim;   /* m_mix::vmix_dim, mix.F90 */
 
 /* ---- callbacks the Fortran needs ---------------------------------------- */
 static int g_n, g_m, g_l, g_periodic;
@@ -322,6 +324,9 @@ int thcmref_init(const thcmref_config *cfg, const int *landm_in, const double *t
     double aT = cfg->alphaT, aS = cfg->alphaS;
     int ih = cfg->ih, vmix = cfg->vmix, tap = cfg->tap, rho = cfg->rho_mixing,
         cor = cfg->coriolis_on, per = cfg->periodic;
+    /* the mixing pattern of a fresh process (THCM is one per process in the
+     * reference; m_mix keeps vmix_dim across init_ calls of this harness) */
+    _QMm_mixEvmix_dim = 0;
     init_(&n, &m, &l, &nml, &xmin, &xmax, &ymin, &ymax, &aT, &aS, &ih, &vmix, &tap, &rho,
           &cor, &per, landm, f[0], f[1], f[2], f[3], f[4]);
     for (int q = 0; q < 5; q++) free(f[q]);
@@ -369,6 +374,9 @@ int thcmref_rhs(const double *x, double intcorr, double *F)
 {
     double *xx = malloc(sizeof(double) * g_ndim);
     memcpy(xx, x, sizeof(double) * g_ndim);
+    /* Ocean::computeRHS calls THCM::fixMixing(0) first (Ocean.C:1281,
+     * THCM.C:2640-2647: only under Mixing = 2) */
+    if (g_cfg.vmix == 2) { int z = 0; _QMm_mixPset_vmix_fix(&z); }
     rhs_(xx, F);
     for (int r = 0; r < g_ndim; r++) F[r] = -F[r];
     if (g_cfg.sres == 0) {
@@ -436,6 +444,7 @@ int thcmref_jacobian(const double *x, int *rowptr, int *col, double *val, double
 {
     double *xx = malloc(sizeof(double) * g_ndim);
     memcpy(xx, x, sizeof(double) * g_ndim);
+    if (g_cfg.vmix == 2) { int z = 0; _QMm_mixPset_vmix_fix(&z); }   /* Ocean.C:1302 */
     matrix_(xx);
     free(xx);
     memcpy(rowptr, g_grow, sizeof(int) * (g_ndim + 1));

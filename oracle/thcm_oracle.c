@@ -56,6 +56,10 @@ struct tho_model {
     double *Frc;                                      /* ndim                     */
     double *Al, *An;                                  /* 27*36 per cell           */
     int vmix_flag, vmix_temp, vmix_salt, vmix_fix;
+    /* the mixing Jacobian's pattern (mix_imp.f:168-227 vmix_part with
+     * vmix_el_1 / vmix_el_2): built at all (vmix_dim > 0), and with the
+     * salinity columns and rows (vmix_el_2 under vmix_salt = 1) */
+    int vmix_part_done, vmix_pat_salt;
     int rowintcon;
     double *coef;
     int *grow, *gcol, gnnz;
@@ -865,15 +869,19 @@ static void mix_rows(const tho_model *h, int i, int j, int k, int pk, int pvar, 
     *mixS = ms;
 }
 
-/* Neutral physics / Gent-McWilliams (MIXP, MKAP) and the adaptive mixing
- * partition (Mixing = 2) are outside this restatement. */
+/* Neutral physics / Gent-McWilliams (MIXP, MKAP) are outside this
+ * restatement. */
 static int mixing_supported(const tho_model *h)
 {
-    return h->par[MIXP] * h->par[PE_H] == 0.0 && h->par[MKAP] * h->par[PE_H] == 0.0
-           && h->c.vmix != 2;
+    return h->par[MIXP] * h->par[PE_H] == 0.0 && h->par[MKAP] * h->par[PE_H] == 0.0;
 }
 
-/* mix_imp.f:131-166 vmix_control (Mixing = 2) */
+/* mix_imp.f:131-166 vmix_control (Mixing = 2): the T / S mixing switches
+ * follow whether the T / S fields are nonzero (l2nrm > 1e-12,
+ * matetc.F90:22-36); when they change and T is on, vmix_part rebuilds the
+ * pattern from vmix_el_2 (mix_imp.f:934-1049) with the new switches (the
+ * reference tests vmix_temp twice there, so a change that leaves T off keeps
+ * the old pattern). */
 static void vmix_control(tho_model *h, const double *un)
 {
     double st = 0.0, ss = 0.0;
@@ -881,8 +889,15 @@ static void vmix_control(tho_model *h, const double *un)
         st = un[r + TT - 1] * un[r + TT - 1] + st;
         ss = un[r + SS - 1] * un[r + SS - 1] + ss;
     }
-    h->vmix_temp = sqrt(st) > 1.0e-12 ? 1 : 0;
-    h->vmix_salt = sqrt(ss) > 1.0e-12 ? 1 : 0;
+    const int tt = sqrt(st) > 1.0e-12 ? 1 : 0, ts = sqrt(ss) > 1.0e-12 ? 1 : 0;
+    if (h->vmix_temp != tt || h->vmix_salt != ts) {
+        h->vmix_temp = tt;
+        h->vmix_salt = ts;
+        if (h->vmix_temp != 0) {
+            h->vmix_part_done = 1;
+            h->vmix_pat_salt = h->vmix_salt;
+        }
+    }
     h->vmix_fix = 1;
 }
 
@@ -903,7 +918,9 @@ static void vmix_fun(tho_model *h, double *mix)
 /* mix_imp.f:729-815 vmix_jac, forward differences (eps = 1e-8).  The entry
  * for row r and pattern column c is (mix_r(x + eps e_c) - mix_r(x)) / eps;
  * only the T,S entries of the cell's own column can be nonzero, the others
- * evaluate to exactly 0 and are not formed. */
+ * evaluate to exactly 0 and are not formed.  The pattern holds the T rows
+ * against T (and against S when it has the salinity part) and, with the
+ * salinity part, the S rows against T and S (vmix_el_1: all four). */
 static void vmix_jac(tho_model *h)
 {
     const double eps = 1.0e-08;
@@ -919,12 +936,13 @@ static void vmix_jac(tho_model *h)
                     if (lm != OCEAN && lm != PERIO) continue;
                     int pos = (dk == 0) ? 5 : (dk < 0 ? 14 : 23);
                     for (int var = 0; var < 2; var++) {
+                        if (var == 1 && !h->vmix_pat_salt) continue;   /* S columns not in the pattern */
                         double pT, pS;
                         mix_rows(h, i, j, k, kk, var, eps, &pT, &pS);
                         double fT = (pT - bT) / eps, fS = (pS - bS) / eps;
                         int col = var ? SS : TT, c = CELL(h, i, j, k);
                         AT(An, c, TT, col, pos) = AT(An, c, TT, col, pos) + fT;
-                        AT(An, c, SS, col, pos) = AT(An, c, SS, col, pos) + fS;
+                        if (h->vmix_pat_salt) AT(An, c, SS, col, pos) = AT(An, c, SS, col, pos) + fS;
                     }
                 }
             }
@@ -1254,6 +1272,7 @@ tho_model *tho_create(const tho_config *cfg, const int *landm, const double *tau
     h->vmix_flag = (cfg->vmix == 0) ? 0 : (cfg->vmix == 1 ? 1 : (cfg->vmix == 2 ? 2 : -1));
     h->vmix_fix = (cfg->vmix == 2) ? 0 : 1;
     h->vmix_temp = h->vmix_salt = (h->vmix_flag == 1) ? 1 : 0;
+    h->vmix_part_done = h->vmix_pat_salt = (h->vmix_flag == 1) ? 1 : 0;
     forcing(h);
     lin(h);
     h->rowintcon = -1;
@@ -1311,9 +1330,11 @@ int tho_mix(tho_model *h, const double *x, double *mix)
 {
     if (!mixing_supported(h)) return -2;
     usol(h, x);
+    if (h->vmix_flag == 2) h->vmix_fix = 0;
     if (h->vmix_flag >= 1) {
         if (h->vmix_fix == 0 && h->vmix_flag >= 2) vmix_control(h, x);
-        vmix_fun(h, mix);
+        if ((h->vmix_temp == 1 || h->vmix_salt == 1) && h->vmix_part_done) vmix_fun(h, mix);
+        else memset(mix, 0, sizeof(double) * h->ndim);
     } else {
         memset(mix, 0, sizeof(double) * h->ndim);
     }
@@ -1331,9 +1352,10 @@ int tho_rhs(tho_model *h, const double *x, double intcorr, double *F)
     nlin(h, 0);
     boundaries(h);
     int bad = fillcolA(h);
+    if (h->vmix_flag == 2) h->vmix_fix = 0;   /* THCM::fixMixing(0) of Ocean::computeRHS (Ocean.C:1281) */
     if (h->vmix_flag >= 1) {
         if (h->vmix_fix == 0 && h->vmix_flag >= 2) vmix_control(h, x);
-        if ((h->vmix_temp == 1 || h->vmix_salt == 1)) vmix_fun(h, mix);
+        if ((h->vmix_temp == 1 || h->vmix_salt == 1) && h->vmix_part_done) vmix_fun(h, mix);
     }
     for (int r = 0; r < nd; r++) {
         double au = 0.0;
@@ -1375,9 +1397,15 @@ int tho_jacobian(tho_model *h, const double *x, double *val, double *diagB)
                 }
     usol(h, x);
     nlin(h, 1);
+    if (h->vmix_flag == 2) h->vmix_fix = 0;   /* THCM::fixMixing(0) of Ocean::computeJacobian (Ocean.C:1302) */
     if (h->vmix_flag >= 1) {
         if (h->vmix_fix == 0 && h->vmix_flag >= 2) vmix_control(h, x);
-        if (h->vmix_temp == 1 || h->vmix_salt == 1) vmix_jac(h);
+        /* A pattern built with a zero S field leaves the S columns out; the
+         * reference perturbs them with whichever T column group its DSM
+         * colouring puts them in (mix_sup.F90), and the T-row entries then
+         * depend on that colouring, which this restatement does not follow. */
+        if (h->vmix_temp == 1 && h->vmix_part_done && !h->vmix_pat_salt) return -2;
+        if ((h->vmix_temp == 1 || h->vmix_salt == 1) && h->vmix_part_done) vmix_jac(h);
     }
     boundaries(h);
     int bad = fillcolA(h);

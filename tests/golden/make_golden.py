@@ -85,6 +85,7 @@ def cases():
     yield Case("global96", cfg, landm, analytic_forcing(cfg),
                dict(COMB=1.0, WIND=1.0, SALT=1.0, TEMP=1.0), rand_state(cfg, 6)), 0.02
     yield reft16_case(), 0.25
+    yield reft16_mix2_case(), 0.25
 
 
 def reft16_case():
@@ -111,8 +112,48 @@ def reft16_case():
     return Case("reft16", cfg, landm, {}, pars, z["x"])
 
 
+def mixing2_sequence():
+    """Mixing = 2 (the adaptive partition of mix_imp.f:131-227) on the natl8
+    basin: one fresh model evaluated, as Ocean::computeRHS and then
+    Ocean::computeJacobian do, at a sequence of states that switches the
+    T / S mixing on and off: T only (pattern built without salinity), T and
+    S (pattern rebuilt), S only (T switched off, pattern kept), neither, T and
+    S again.  The T-only Jacobian is stored as the reference gives it; its
+    entries depend on the reference's DSM column grouping."""
+    cfg = make_config(n=8, m=8, l=4, rho_mixing=0, sres=0, vmix=2)
+    landm = RefModel.reference_landm(cfg, 0, 1, "mask_natl8")
+    case = Case("natl8_mix2", cfg, landm, {}, dict(COMB=0.7, WIND=1.0, SALT=1.0, TEMP=10.0), rand_state(cfg, 8))
+    x = case.x
+    var = np.arange(x.size) % 6
+    states = [("t_only", np.where(var == 5, 0.0, x)), ("t_and_s", x), ("s_only", np.where(var == 4, 0.0, x)),
+              ("neither", np.where((var == 4) | (var == 5), 0.0, x)), ("t_and_s_again", 0.5 * x)]
+    ref = RefModel(case)
+    out = dict(cfg=json.dumps(config_dict(cfg)), pars=json.dumps(case.pars), landm=landm,
+               intcorr=np.float64(case.intcorr), labels=np.array([k for k, _ in states]))
+    for q, (lab, xs) in enumerate(states):
+        out[f"x{q}"] = xs
+        out[f"F{q}"] = ref.rhs(xs)
+        rp, ci, val, dB, dropped = ref.jacobian(xs)
+        assert dropped == 0
+        out[f"J{q}"] = val
+    out["J_rowptr"], out["J_col"] = rp, ci
+    path = os.path.join(HERE, "mix2", "natl8_sequence.npz")
+    np.savez_compressed(path, **out)
+    print(f"{path}: {len(states)} states, size={os.path.getsize(path)} B")
+
+
+def reft16_mix2_case():
+    """reft16 with the reference test's own Mixing = 2."""
+    c = reft16_case()
+    c.cfg.vmix = 2
+    c.name = "reft16_mix2"
+    return c
+
+
 if __name__ == "__main__":
     only = set(sys.argv[1:])   # case names to write (all when none are given)
     for case, frac in cases():
         if not only or case.name in only:
             write(case, frac)
+    if not only or "natl8_mix2" in only:
+        mixing2_sequence()

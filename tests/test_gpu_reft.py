@@ -7,10 +7,9 @@ with those of the state it stores (ocean_reference.h5) to 1e-3
 (reft_ocean.C:59-89).  The same check runs here on the library's device
 continuation.  Also: Newton from the stored state itself.
 
-The reference runs Mixing = 2; its right-hand side equals that of
-Mixing = 1 once T and S are nonzero, which the fixture uses (its Jacobian's
-finite-difference mixing block is grouped differently, vmix_el_1 against
-vmix_el_2, which changes Newton's path, not the steady states)."""
+The reference runs Mixing = 2 (the adaptive mixing partition): at rest the
+mixing is off, and once T and S are nonzero F and J equal those of
+Mixing = 1 (fixtures reft16_mix2 and reft16).  The continuation runs both."""
 import os
 
 import numpy as np
@@ -51,14 +50,16 @@ def test_gpu_newton_from_the_stored_reference_state():
     assert np.all(d < 1e-4)
 
 
-def test_gpu_reproduces_reft_ocean():
+@pytest.mark.parametrize("name", ["reft16_mix2", "reft16"])
+def test_gpu_reproduces_reft_ocean(name):
     """Continuation from rest to COMB = 0.02 with the reference test's
     parameters (initial step 2e-3, steps 1e-8 .. 1e-1, at most 20 steps,
     3 optimal Newton iterations, Newton tolerance 1e-2, dF/dpar increment
     1e-5, state tangent scaling 1e-3, predictor bound 3000), then the test's
     own check: each of the u, v, T, S norms within 1e-3 of the stored
-    state's."""
-    case, _ = load_golden("reft16")
+    state's.  reft16_mix2 is the test's configuration as written (Mixing =
+    2); reft16 the same with Mixing = 1."""
+    case, _ = load_golden(name)
     pars = dict(case.pars)
     pars["COMB"] = 0.0   # Starting Parameters of reft_ocean_params.xml
     start = Case("reft16_rest", case.cfg, case.landm, case.fields, pars, np.zeros(case.ndim))

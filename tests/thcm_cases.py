@@ -195,6 +195,15 @@ class OracleModel:
         assert rc >= 0, rc
         return rp, ci, val, dB, rc
 
+    def jacobian_rc(self, x=None):
+        """jacobian() without the success check: (rowptr, col, val, diagB, rc)."""
+        x = self.case.x if x is None else np.ascontiguousarray(x, np.float64)
+        rp, ci = self.graph()
+        val = np.zeros(len(ci))
+        dB = np.zeros(self.ndim)
+        rc = self.lib.tho_jacobian(self.h, _ptr(x, C.c_double), _ptr(val, C.c_double), _ptr(dB, C.c_double))
+        return rp, ci, val, dB, rc
+
     def mix(self, x=None):
         x = self.case.x if x is None else np.ascontiguousarray(x, np.float64)
         out = np.zeros(self.ndim)
