@@ -29,6 +29,7 @@ struct KArgs {
     Scalars s;
     const uint8_t *lm;
     const double *al, *jt, *kt, *frc;
+    double *icrow;   // the integral row's own S-equation entries (20 slots), or null
 };
 
 // ------------------------------------------------------------------ helpers
@@ -761,6 +762,9 @@ __global__ void __launch_bounds__(256) k_jacobian(KArgs g, const double *__restr
             }
     bnd_ts(b, a, 12);
     if (6 * cell + S == g.s.rowintcon) {   // replaced by the integral row
+        if (g.icrow)   // the Fortran S equation of this row, kept for the Fortran-layout export
+#pragma unroll
+            for (int s = 0; s < 20; s++) g.icrow[s] = (fabs(a[s]) > 1.0e-10) ? a[s] : 0.0;
 #pragma unroll
         for (int s = 0; s < 20; s++) a[s] = 0.0;
     }
@@ -1007,6 +1011,7 @@ static KArgs kargs(iemic_ctx *c)
     g.jt = c->d_jt;
     g.kt = c->d_kt;
     g.frc = c->d_frc;
+    g.icrow = c->d_icrow;
     return g;
 }
 
@@ -1027,6 +1032,7 @@ int upload_tables(iemic_ctx *c)
         HIPCHK(hipMalloc(&c->d_diagB, h.diagB.size() * sizeof(double)));
         c->nred = 1024;
         HIPCHK(hipMalloc(&c->d_red, (c->nred + 8) * sizeof(double)));
+        if (c->sc.rowintcon >= 0) HIPCHK(hipMalloc(&c->d_icrow, 20 * sizeof(double)));
         HIPCHK(hipMemcpy(c->d_coef, h.coef.data(), h.coef.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(c->d_al, h.al.data(), h.al.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -1158,6 +1164,76 @@ int launch_rhs_partials(iemic_ctx *c, const double *d_x, double *d_F, double *d_
                            c->intcorr, c->sc.rowintcon);
         HIPCHK(hipGetLastError());
     }
+    return IEMIC_OK;
+}
+
+// rhs_ (usrc.F90:523-603) in the Fortran's own convention: B = -F, every
+// row the model equation (the integral row keeps its S equation).
+int launch_rhs_fortran(iemic_ctx *c, const double *d_x, double *d_B)
+{
+    if (int rc = mixing_control(c, d_x, c->d_red, c->nred)) return rc;
+    KArgs g = kargs(c);
+    hipLaunchKernelGGL(k_rhs, dim3(nblk(c->sc.ncell, 256)), dim3(256), 0, c->stream, g, d_x, d_B);
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+
+// The Jacobian in the layout fillcolA writes (assemble.F90:57-139): rows in
+// order, each row's entries in stencil position then unknown order -- the
+// order of its slots -- keeping |a| > 1e-10 (every nonzero slot value),
+// 1-based.  The integral row holds its S equation (the Fortran matrix;
+// THCM.C replaces that row only in its own copy).
+__device__ __forceinline__ double fort_value(const Scalars &s, const double *J, const double *icrow, int cell, int sl)
+{
+    if (icrow && 6 * cell + S == s.rowintcon && SLOTS[sl].eq == S) return icrow[sl - ROW0[S]];
+    return J[(size_t)sl * s.ncell + cell];
+}
+__global__ void __launch_bounds__(256) k_fort_count(Scalars s, const double *__restrict__ J,
+                                                    const double *__restrict__ icrow, int *__restrict__ cnt)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.ndim) return;
+    const int cell = r / 6, e = r % 6;
+    int c = 0;
+    for (int sl = ROW0[e]; sl < ROW0[e + 1]; sl++) c += fort_value(s, J, icrow, cell, sl) != 0.0;
+    cnt[r] = c;
+}
+__global__ void __launch_bounds__(256) k_fort_fill(Scalars s, const double *__restrict__ J,
+                                                   const double *__restrict__ icrow, const int *__restrict__ beg,
+                                                   int *__restrict__ col, double *__restrict__ val,
+                                                   int *__restrict__ bad)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.ndim) return;
+    const int cell = r / 6, e = r % 6;
+    const int i = cell % s.n + 1, j = (cell / s.n) % s.m + 1, k = cell / (s.n * s.m) + 1;
+    int q = beg[r];
+    for (int sl = ROW0[e]; sl < ROW0[e + 1]; sl++) {
+        const double v = fort_value(s, J, icrow, cell, sl);
+        if (v == 0.0) continue;
+        int nb;
+        if (!nb_cell(s, i, j, k, SLOTS[sl].kk, nb)) {   // a value outside the grid cannot be placed
+            atomicAdd(bad, 1);
+            nb = cell;
+        }
+        col[q] = 6 * nb + SLOTS[sl].var + 1;
+        val[q] = v;
+        q++;
+    }
+}
+
+int launch_fortran_count(iemic_ctx *c, int *d_cnt)
+{
+    hipLaunchKernelGGL(k_fort_count, dim3(nblk(c->sc.ndim, 256)), dim3(256), 0, c->stream, c->sc, c->d_J,
+                       c->d_icrow, d_cnt);
+    HIPCHK(hipGetLastError());
+    return IEMIC_OK;
+}
+int launch_fortran_fill(iemic_ctx *c, const int *d_beg, int *d_col, double *d_val, int *d_bad)
+{
+    hipLaunchKernelGGL(k_fort_fill, dim3(nblk(c->sc.ndim, 256)), dim3(256), 0, c->stream, c->sc, c->d_J,
+                       c->d_icrow, d_beg, d_col, d_val, d_bad);
+    HIPCHK(hipGetLastError());
     return IEMIC_OK;
 }
 

@@ -4,6 +4,7 @@
 // solver.hip.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -92,7 +93,7 @@ int iemic_destroy(iemic_ctx *c)
     precond_free(c);
     krylov_free(c);
     void *bufs[] = {c->d_landm, c->d_al, c->d_jt, c->d_kt, c->d_frc, c->d_coef, c->d_diagB,
-                    c->d_J, c->d_x, c->d_y, c->d_red, c->d_src, c->d_col_export};
+                    c->d_J, c->d_x, c->d_y, c->d_red, c->d_src, c->d_col_export, c->d_icrow};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &ev : c->ev)
@@ -211,6 +212,105 @@ int iemic_jacobian(iemic_ctx *c, const double *x)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     return IEMIC_OK;
+}
+
+// ---- the Fortran interface of THCM.C:49-176 (usrc.F90, mat.F90) --------
+int iemic_thcm_get_array_sizes(const iemic_ctx *c, int *nrows, int *nnz)
+{
+    CHECK_CTX(c);
+    if (!nrows || !nnz) return set_error(IEMIC_EINVAL, "null pointer");
+    *nrows = c->sc.ndim;
+    const long long cap = (long long)c->sc.ndim * (6 * 27 + 1);   // ndim (nun np + 1), mat.F90:58-70
+    if (cap > INT32_MAX) return set_error(IEMIC_EUNSUPPORTED, "Fortran CSR capacity exceeds 32-bit indices");
+    *nnz = (int)cap;
+    return IEMIC_OK;
+}
+
+int iemic_thcm_set_pointers(iemic_ctx *c, int nrows, int nnz, int *begA, int *jcoA, double *coA, double *coB,
+                            int *begF, int *jcoF, double *coF)
+{
+    CHECK_CTX(c);
+    (void)begF; (void)jcoF; (void)coF;   // the stochastic-forcing matrix (get_stochastic_forcing) is not filled
+    if (nrows != c->sc.ndim) return set_error(IEMIC_EINVAL, "set_pointers: nrows differs from the model's ndim");
+    if (nnz < 1 || !begA || !jcoA || !coA || !coB) return set_error(IEMIC_EINVAL, "set_pointers: null or empty buffer");
+    c->f_nrows = nrows;
+    c->f_nnz = nnz;
+    c->f_begA = begA;
+    c->f_jcoA = jcoA;
+    c->f_coA = coA;
+    c->f_coB = coB;
+    return IEMIC_OK;
+}
+
+int iemic_thcm_rhs(iemic_ctx *c, const double *un, double *B)
+{
+    CHECK_CTX(c);
+    if (!un || !B) return set_error(IEMIC_EINVAL, "null pointer");
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    const size_t bytes = (size_t)c->sc.ndim * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_x, un, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = launch_rhs_fortran(c, c->d_x, c->d_y))) return rc;
+    HIPCHK(hipMemcpyAsync(B, c->d_y, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < c->sc.ndim; r++) B[r] = -B[r];   // F = -B (THCM.C:1011)
+    return IEMIC_OK;
+}
+
+int iemic_thcm_matrix(iemic_ctx *c, const double *un)
+{
+    CHECK_CTX(c);
+    if (!un) return set_error(IEMIC_EINVAL, "null pointer");
+    if (!c->f_begA) return set_error(IEMIC_ESTATE, "matrix: call iemic_thcm_set_pointers first");
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    const int nd = c->sc.ndim;
+    HIPCHK(hipMemcpyAsync(c->d_x, un, (size_t)nd * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = launch_jacobian(c, c->d_x))) return rc;
+    int *d_cnt = nullptr, *d_col = nullptr, *d_bad = nullptr;
+    double *d_val = nullptr;
+    std::vector<int> beg0(nd + 1);
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(c->stream);
+        for (void *b : {(void *)d_cnt, (void *)d_col, (void *)d_bad, (void *)d_val})
+            if (b) (void)hipFree(b);
+        return code;
+    };
+    if (hipMalloc(&d_cnt, (size_t)(nd + 1) * sizeof(int)) != hipSuccess || hipMalloc(&d_bad, sizeof(int)) != hipSuccess)
+        return done(set_error(IEMIC_ENOMEM, "matrix: staging"));
+    if ((rc = launch_fortran_count(c, d_cnt))) return done(rc);
+    if (hipMemcpyAsync(beg0.data(), d_cnt, (size_t)nd * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return done(set_error(IEMIC_EHIP, "matrix: row counts"));
+    long long tot = 0;
+    for (int r = 0; r < nd; r++) {
+        const int cnt = beg0[r];
+        beg0[r] = (int)tot;
+        tot += cnt;
+    }
+    beg0[nd] = (int)tot;
+    if (tot > c->f_nnz) return done(set_error(IEMIC_EINVAL, "matrix: the caller's CSR buffers are too small"));
+    for (int r = 0; r <= nd; r++) c->f_begA[r] = beg0[r] + 1;   // 1-based, as fillcolA writes begA
+    if (hipMalloc(&d_col, (size_t)std::max<long long>(tot, 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_val, (size_t)std::max<long long>(tot, 1) * sizeof(double)) != hipSuccess)
+        return done(set_error(IEMIC_ENOMEM, "matrix: staging"));
+    int bad = 0;
+    if (hipMemcpyAsync(d_cnt, beg0.data(), (size_t)(nd + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess ||
+        hipMemsetAsync(d_bad, 0, sizeof(int), c->stream) != hipSuccess)
+        return done(set_error(IEMIC_EHIP, "matrix: offsets"));
+    if ((rc = launch_fortran_fill(c, d_cnt, d_col, d_val, d_bad))) return done(rc);
+    if (hipMemcpyAsync(c->f_jcoA, d_col, (size_t)tot * sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->f_coA, d_val, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return done(set_error(IEMIC_EHIP, "matrix: copy back"));
+    if (bad) return done(set_error(IEMIC_EINVAL, "matrix: entries outside the grid"));
+    // fillcolB (assemble.F90:18-54): THCM's diag(B) except the integral
+    // row, which THCM zeroes in its own copy (THCM.C:2197-2198)
+    std::memcpy(c->f_coB, c->h.diagB.data(), (size_t)nd * sizeof(double));
+    if (c->sc.rowintcon >= 0) c->f_coB[c->sc.rowintcon] = -1.0;
+    return done(IEMIC_OK);
 }
 
 int iemic_jacobian_csr(iemic_ctx *c, int *rowptr, int *col, double *val, double *diagB)

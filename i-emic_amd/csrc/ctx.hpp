@@ -62,6 +62,11 @@ struct iemic_ctx {
     int nred = 0;
     int *d_src = nullptr;            // CSR export map (slot index per entry)
     int *d_col_export = nullptr;     // CSR export columns
+    double *d_icrow = nullptr;       // the integral row's S-equation entries (Fortran-layout export)
+    // caller buffers of the Fortran-layout interface (m_mat::set_pointers)
+    int f_nrows = 0, f_nnz = 0;
+    int *f_begA = nullptr, *f_jcoA = nullptr;
+    double *f_coA = nullptr, *f_coB = nullptr;
     bool have_jac = false;
     unsigned jac_version = 0;        // bumped by every Jacobian assembly
     int prec_kind = IEMIC_PREC_ROW_LU;
@@ -91,6 +96,9 @@ int launch_rhs(iemic_ctx *c, const double *d_x, double *d_F);
 int launch_rhs_partials(iemic_ctx *c, const double *d_x, double *d_F, double *d_part, int cap);
 int mixing_control(iemic_ctx *c, const double *d_x, double *d_part, int cap);
 int launch_jacobian(iemic_ctx *c, const double *d_x);
+int launch_rhs_fortran(iemic_ctx *c, const double *d_x, double *d_B);
+int launch_fortran_count(iemic_ctx *c, int *d_cnt);
+int launch_fortran_fill(iemic_ctx *c, const int *d_beg, int *d_col, double *d_val, int *d_bad);
 int launch_spmv(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_spmv_t(iemic_ctx *c, const double *d_x, double *d_y);
 int launch_spmm(iemic_ctx *c, int nrhs, const double *d_X, size_t ldx, double *d_Y, size_t ldy);

@@ -107,6 +107,10 @@ def load():
         "iemic_rhs": ([vp, dp, dp], C.c_int),
         "iemic_rhs_dev": ([vp, vp, vp], C.c_int),
         "iemic_rhs_dev_noalloc": ([vp, vp, vp], C.c_int),
+        "iemic_thcm_get_array_sizes": ([vp, ip, ip], C.c_int),
+        "iemic_thcm_set_pointers": ([vp, C.c_int, C.c_int, ip, ip, dp, dp, ip, ip, dp], C.c_int),
+        "iemic_thcm_rhs": ([vp, dp, dp], C.c_int),
+        "iemic_thcm_matrix": ([vp, dp], C.c_int),
         "iemic_jacobian": ([vp, dp], C.c_int),
         "iemic_jacobian_dev": ([vp, vp], C.c_int),
         "iemic_jacobian_csr": ([vp, ip, ip, dp, dp], C.c_int),
@@ -473,6 +477,30 @@ class Ocean:
     def computeRHS_dev_noalloc(self, dx, dF):
         """F(x) into dF with no device allocation (iemic_rhs_dev_noalloc)."""
         _check(self.lib.iemic_rhs_dev_noalloc(self.h, dx.ptr, dF.ptr))
+
+    def fortranRHS(self, un):
+        """rhs_: the Fortran right-hand side B (iemic_thcm_rhs)."""
+        un = _f64(un)
+        B = np.empty(self.ndim)
+        _check(self.lib.iemic_thcm_rhs(self.h, _d(un), _d(B)))
+        return B
+
+    def fortranMatrix(self, un):
+        """matrix_: (begA, jcoA, coA, coB) in the Fortran's 1-based CSR layout,
+        through iemic_thcm_get_array_sizes / set_pointers / matrix."""
+        un = _f64(un)
+        nr, nz = C.c_int(), C.c_int()
+        _check(self.lib.iemic_thcm_get_array_sizes(self.h, C.byref(nr), C.byref(nz)))
+        beg = np.zeros(nr.value + 1, np.int32)
+        jco = np.zeros(nz.value, np.int32)
+        co = np.zeros(nz.value)
+        coB = np.zeros(nr.value)
+        ip = _P(C.c_int)
+        _check(self.lib.iemic_thcm_set_pointers(self.h, nr, nz, beg.ctypes.data_as(ip), jco.ctypes.data_as(ip),
+                                                _d(co), _d(coB), None, None, None))
+        _check(self.lib.iemic_thcm_matrix(self.h, _d(un)))
+        n = beg[-1] - 1
+        return beg, jco[:n].copy(), co[:n].copy(), coB
 
     def computeJacobian_dev(self, dx):
         _check(self.lib.iemic_jacobian_dev(self.h, dx.ptr))

@@ -145,6 +145,31 @@ int iemic_jacobian_dev(iemic_ctx *ctx, const double *d_x);
  * ndim+1, graph_nnz, graph_nnz, ndim. */
 int iemic_jacobian_csr(iemic_ctx *ctx, int *rowptr, int *col, double *val, double *diagB);
 
+/* ---- the Fortran interface THCM.C binds (THCM.C:49-176; usrc.F90, mat.F90)
+ * A THCM.C that links this library in place of the THCM Fortran calls these
+ * where it calls m_mat::get_array_sizes, m_mat::set_pointers, rhs_ and
+ * matrix_ (THCM.C:617-638, 1002, 1056); host arrays, 1-based CSR. */
+/* m_mat::get_array_sizes (mat.F90:58-70): nrows = ndim, nnz = ndim (6 27 + 1),
+ * the capacity THCM.C allocates (THCM.C:617-630). */
+int iemic_thcm_get_array_sizes(const iemic_ctx *ctx, int *nrows, int *nnz);
+/* m_mat::set_pointers (mat.F90:75-104): caller-allocated buffers that
+ * iemic_thcm_matrix fills: begA (nrows + 1), jcoA and coA (nnz), coB (nrows).
+ * The stochastic-forcing matrix begF / jcoF / coF (get_stochastic_forcing,
+ * forcing.F90:236-280) is accepted and not written (may be NULL). */
+int iemic_thcm_set_pointers(iemic_ctx *ctx, int nrows, int nnz, int *begA, int *jcoA, double *coA, double *coB,
+                            int *begF, int *jcoF, double *coF);
+/* rhs_ (usrc.F90:523-603): B = -A(u) u - mix + Frc with land rows zeroed, the
+ * Fortran's own sign and no integral condition (iemic_rhs returns F = -B with
+ * it, THCM.C:1011-1027).  un and B: ndim doubles. */
+int iemic_thcm_rhs(iemic_ctx *ctx, const double *un, double *B);
+/* matrix_ (usrc.F90:449-521): J(un) into the set_pointers buffers as
+ * fillcolA writes it (assemble.F90:57-139): rows in order, each row's entries
+ * by stencil position then unknown, |a| > 1e-10 kept, 1-based begA / jcoA;
+ * the integral-condition row holds its S equation.  coB = fillcolB
+ * (assemble.F90:18-54).  J also becomes the context's Jacobian, as
+ * iemic_jacobian would make it. */
+int iemic_thcm_matrix(iemic_ctx *ctx, const double *un);
+
 /* ---- Ocean::applyMatrix / applyMassMat (Ocean.C:1369-1374, 1448-1457) -- */
 int iemic_apply_matrix(iemic_ctx *ctx, const double *v, double *out);
 int iemic_apply_matrix_dev(iemic_ctx *ctx, const double *d_v, double *d_out);

@@ -428,6 +428,22 @@ double thcmref_psim(const double *x, double *psim)
     return mx;
 }
 
+/* rhs_ itself (usrc.F90:523-603): the Fortran right-hand side B, not negated
+ * and without the integral row, as the C ABI of THCM.C:49-176 receives it. */
+int thcmref_fortran_rhs(const double *x, double *B)
+{
+    double *xx = malloc(sizeof(double) * g_ndim);
+    memcpy(xx, x, sizeof(double) * g_ndim);
+    if (g_cfg.vmix == 2) { int z = 0; _QMm_mixPset_vmix_fix(&z); }
+    rhs_(xx, B);
+    free(xx);
+    return 0;
+}
+
+/* coB of the last matrix_ call (fillcolB, assemble.F90:18-54), as written
+ * into the caller's buffer by set_pointers. */
+void thcmref_fortran_coB(double *out) { memcpy(out, g_coB, sizeof(double) * g_ndim); }
+
 /* The Fortran CSR of the last rhs_/matrix_ call (1-based), for inspection. */
 int thcmref_fortran_csr_nnz(void) { return g_begA[g_ndim] - 1; }
 void thcmref_fortran_csr(int *beg, int *jco, double *co)

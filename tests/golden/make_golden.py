@@ -142,6 +142,26 @@ def mixing2_sequence():
     print(f"{path}: {len(states)} states, size={os.path.getsize(path)} B")
 
 
+def fortran_layout():
+    """The Fortran interface itself on the natl8 and gateway16 fixtures:
+    rhs_'s B, and after matrix_ the caller-allocated buffers that set_pointers
+    registered (1-based begA / jcoA / coA in fillcolA's order, coB)."""
+    for name in ("natl8", "gateway16"):
+        case, _ = load_golden_case(name)
+        ref = RefModel(case)
+        B = ref.fortran_rhs()
+        ref.jacobian()
+        beg, jco, co = ref.fortran_csr()
+        path = os.path.join(HERE, "fortran", name + ".npz")
+        np.savez_compressed(path, B=B, begA=beg, jcoA=jco, coA=co, coB=ref.fortran_coB())
+        print(f"{path}: nnz={len(co)} size={os.path.getsize(path)} B")
+
+
+def load_golden_case(name):
+    from thcm_cases import load_golden
+    return load_golden(name)
+
+
 def reft16_mix2_case():
     """reft16 with the reference test's own Mixing = 2."""
     c = reft16_case()
@@ -157,3 +177,5 @@ if __name__ == "__main__":
             write(case, frac)
     if not only or "natl8_mix2" in only:
         mixing2_sequence()
+    if not only or "fortran" in only:
+        fortran_layout()

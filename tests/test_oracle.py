@@ -2,6 +2,7 @@
 vectors and against what the mathematics fixes.  CPU only."""
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from thcm_cases import (OCEAN, LAND, PAR, Case, OracleModel, box_mask, box16_case,
                         expected_rows, golden_names, load_golden, make_config, restrict_rows)
@@ -168,3 +169,30 @@ def test_stored_state_of_reft_ocean_is_a_steady_state():
     cfg0.vmix = 0
     F0 = OracleModel(Case("reft16_nomix", cfg0, case.landm, case.fields, case.pars, case.x)).rhs()
     assert np.linalg.norm(F0) > 0.1
+
+
+@pytest.mark.parametrize("name", ["natl8", "gateway16"])
+def test_fortran_layout_fixture_is_the_jacobian_fixture(name):
+    """The reference's own Fortran buffers (tests/golden/fortran) hold the
+    fixture's Jacobian: scattered into the fixed graph they equal J in every
+    row but the integral row (THCM.C:1082-1169 copies them there), whose
+    Fortran row is the S equation; B = -F in every row but that one; coB
+    equals diag(B) but at the integral row (-1, an ocean S row, which THCM
+    zeroes)."""
+    import os
+    from thcm_cases import GOLDEN
+    case, exp = load_golden(name)
+    z = np.load(os.path.join(GOLDEN, "fortran", name + ".npz"), allow_pickle=False)
+    nd = case.ndim
+    ric = int(exp["rowintcon"])
+    beg, jco, co = z["begA"] - 1, z["jcoA"] - 1, z["coA"]
+    rows = np.repeat(np.arange(nd), np.diff(beg))
+    J = sp.csr_matrix((co, (rows, jco)), shape=(nd, nd))
+    G = sp.csr_matrix((exp["J_val"], exp["J_col"], exp["J_rowptr"]), shape=(nd, nd))
+    keep = np.arange(nd) != ric
+    D = (J - G).tocsr()[keep]
+    assert D.nnz == 0 or np.abs(D.data).max() == 0.0
+    assert np.array_equal(-z["B"][keep], exp["F"][keep])
+    assert np.array_equal(z["coB"][keep], exp["diagB"][keep]) and z["coB"][ric] == -1.0
+    # each row's entries in stencil-position order, so not sorted by column
+    assert np.all(np.abs(co) > 1e-10)

@@ -102,6 +102,8 @@ def load_ref():
                                          _P(C.c_double), _P(C.c_double)]
         lib.thcmref_fortran_csr.argtypes = [_P(C.c_int), _P(C.c_int), _P(C.c_double)]
         lib.thcmref_psim.argtypes = [_P(C.c_double), _P(C.c_double)]
+        lib.thcmref_fortran_rhs.argtypes = [_P(C.c_double), _P(C.c_double)]
+        lib.thcmref_fortran_coB.argtypes = [_P(C.c_double)]
         lib.thcmref_psim.restype = C.c_double
         _ref_lib = lib
     return _ref_lib
@@ -283,6 +285,18 @@ class RefModel:
         out = np.zeros((c.m + 1) * (c.l + 1))
         mx = on_big_stack(self.lib.thcmref_psim, _ptr(x, C.c_double), _ptr(out, C.c_double))
         return mx, out
+
+    def fortran_rhs(self, x=None):
+        """rhs_ itself: the Fortran right-hand side B (usrc.F90:523-603)."""
+        x = self.case.x if x is None else np.ascontiguousarray(x, np.float64)
+        B = np.zeros(self.ndim)
+        on_big_stack(self.lib.thcmref_fortran_rhs, _ptr(x, C.c_double), _ptr(B, C.c_double))
+        return B
+
+    def fortran_coB(self):
+        out = np.zeros(self.ndim)
+        self.lib.thcmref_fortran_coB(_ptr(out, C.c_double))
+        return out
 
     def fortran_csr(self):
         nz = self.lib.thcmref_fortran_csr_nnz()
