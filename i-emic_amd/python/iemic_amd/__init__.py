@@ -108,6 +108,13 @@ def load():
         "iemic_rhs_dev": ([vp, vp, vp], C.c_int),
         "iemic_rhs_dev_noalloc": ([vp, vp, vp], C.c_int),
         "iemic_thcm_get_array_sizes": ([vp, ip, ip], C.c_int),
+        "iemic_mrilu_create": ([ip, C.c_int, C.c_int, ip, ip, dp], C.c_int),
+        "iemic_mrilu_set_params": ([C.c_int] * 6 + [C.c_double] * 3 + [C.c_int] + [C.c_double] * 7 + [C.c_int]
+                                   + [C.c_double] * 4 + [C.c_int] * 2, C.c_int),
+        "iemic_mrilu_compute": ([C.c_int], C.c_int),
+        "iemic_mrilu_apply": ([C.c_int, C.c_int, dp, dp], C.c_int),
+        "iemic_mrilu_info": ([C.c_int, ip, ip, ip], C.c_int),
+        "iemic_mrilu_destroy": ([ip], C.c_int),
         "iemic_thcm_set_pointers": ([vp, C.c_int, C.c_int, ip, ip, dp, dp, ip, ip, dp], C.c_int),
         "iemic_thcm_rhs": ([vp, dp, dp], C.c_int),
         "iemic_thcm_matrix": ([vp, dp], C.c_int),
@@ -230,6 +237,58 @@ def write_state(path, x, pars=None):
             arr[i] = v
         pp = arr.ctypes.data_as(_P(C.c_double))
     _check(load().iemic_write_state(str(path).encode(), x.ctypes.data_as(_P(C.c_double)), x.size, pp))
+
+
+class MriluPrec:
+    """The MRILU plug-in interface (iemic_mrilu_*): create from a CSR matrix,
+    compute, apply, destroy; block ILU(0) on the GPU."""
+
+    DEFAULTS = dict(blocksize=1, cutmck=0, scarow=0, xactelm=0, clsonce=0, nlsfctr=1.0, epsw=1.0e-2,
+                    elmfctr=0.1, gusmod=0, gusfctr=1.0, redfctr=0.8, schtol=1.0e-6, denslim=1.0e-3,
+                    globfrac=1.0, locfrac=1.0, sparslim=0.675, ilutype=9, droptol=1.0e-4, compfct=1.0,
+                    cpivtol=0.1, lutol=1.0e-14, singlu=1, outlev=2)
+
+    def __init__(self, A, **params):
+        """A: a scipy.sparse matrix (any format, converted to CSR)."""
+        import scipy.sparse as sp
+        self.lib = load()
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        self.n = A.shape[0]
+        self.id = C.c_int(0)
+        beg = A.indptr.astype(np.int32)
+        jco = A.indices.astype(np.int32)
+        co = A.data.astype(np.float64)
+        ip = _P(C.c_int)
+        _check(self.lib.iemic_mrilu_create(C.byref(self.id), self.n, len(co), beg.ctypes.data_as(ip),
+                                           jco.ctypes.data_as(ip), _d(co)))
+        p = dict(self.DEFAULTS)
+        p.update(params)
+        _check(self.lib.iemic_mrilu_set_params(self.id.value, *[p[k] for k in self.DEFAULTS]))
+
+    def compute(self):
+        _check(self.lib.iemic_mrilu_compute(self.id.value))
+
+    def apply(self, v):
+        v = _f64(v)
+        out = np.empty(self.n)
+        _check(self.lib.iemic_mrilu_apply(self.id.value, self.n, _d(v), _d(out)))
+        return out
+
+    def info(self):
+        b, lv, fl = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib.iemic_mrilu_info(self.id.value, C.byref(b), C.byref(lv), C.byref(fl)))
+        return b.value, lv.value, fl.value
+
+    def close(self):
+        if self.id.value:
+            self.lib.iemic_mrilu_destroy(C.byref(self.id))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceArray:

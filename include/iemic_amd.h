@@ -252,6 +252,40 @@ int iemic_solve_multi_transpose_dev(iemic_ctx *ctx, const iemic_solver_params *p
 /* out = M^{-1} in on nrhs columns (FP64 factors) */
 int iemic_apply_precon_multi_dev(iemic_ctx *ctx, int nrhs, const double *d_in, double *d_out);
 
+/* ---- the preconditioner plug-in of the reference's MRILU interface -------
+ * (mrilucpp.F90:120-554, declared for C++ in Ifpack_MRILU.cpp:25-37): an
+ * integer handle per instance (id = 0 on create: a new one; id > 0: that
+ * instance overwritten), the matrix handed over as 0-based CSR (copied),
+ * set_params, compute (once per matrix), apply (before compute: sol = rhs,
+ * with a warning), destroy (sets id = 0).  Behind it runs a GPU block ILU(0)
+ * with level-scheduled sweeps over blocks of `blocksize` (1..8) rows, not
+ * MRILU's multilevel renumbering: blocksize and lutol (pivot floor relative
+ * to the block's largest entry) are used, the other parameters are kept and
+ * have no effect.  Host vectors. */
+int iemic_mrilu_create(int *id, int n, int nnz, const int *beg, const int *jco, const double *co);
+int iemic_mrilu_set_params(int id, int blocksize, int cutmck, int scarow, int xactelm, int clsonce, double nlsfctr,
+                           double epsw, double elmfctr, int gusmod, double gusfctr, double redfctr, double schtol,
+                           double denslim, double globfrac, double locfrac, double sparslim, int ilutype,
+                           double droptol, double compfct, double cpivtol, double lutol, int singlu, int outlev);
+int iemic_mrilu_compute(int id);
+int iemic_mrilu_apply(int id, int n, const double *rhs, double *sol);
+/* block entries, forward dependency levels and floored pivots of the factors */
+int iemic_mrilu_info(int id, int *blocks, int *levels, int *pivots_floored);
+int iemic_mrilu_destroy(int *id);
+/* The same under the reference's own names and argument passing (every
+ * scalar by address, Ifpack_MRILU.cpp:25-37), so that Ifpack_MRILU.cpp links
+ * unchanged; as the Fortran's `stop`, an error prints its message and ends
+ * the process. */
+void mrilucpp_create(int *id, int *n, int *nnz, int *beg, int *jco, double *co);
+void mrilucpp_destroy(int *id);
+void mrilucpp_set_params(const int *id, int *blocksize, int *cutmck, int *scarow, int *xactelm, int *clsonce,
+                         double *nlsfctr, double *epsw, double *elmfctr, int *gusmod, double *gusfctr, double *redfctr,
+                         double *schtol, double *denslim, double *globfrac, double *locfrac, double *sparslim,
+                         int *ilutype, double *droptol, double *compfct, double *cpivtol, double *lutol, int *singlu,
+                         int *outlev);
+void mrilucpp_compute(const int *id);
+void mrilucpp_apply(const int *id, int *dim, const double *rhs, double *sol);
+
 /* ---- callers of the path on device state -------------------------------
  * Damped Newton on F(x) = 0 (the corrector of Continuation::newtonCorrector,
  * Continuation.H:611-790, without the arclength row): x -= t dx, J dx = F,
