@@ -135,7 +135,6 @@ def test_ats_block_of_a_thcm_jacobian():
     J = sp.csr_matrix((val, ci, rp), shape=(oc.ndim, oc.ndim))
     ts = np.nonzero(np.isin(np.arange(oc.ndim) % 6, (4, 5)))[0]
     ric = oc.rowintcon()
-    ts = ts[ts != ric - 1] if False else ts
     A = J[ts][:, ts].tocsr()
     if ric >= 0:   # the dense integral row is not part of ATS in TRIOS: its S equation's diagonal stands in
         k = int(np.nonzero(ts == ric)[0][0])
