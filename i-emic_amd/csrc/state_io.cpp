@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "ctx.hpp"
@@ -68,6 +69,7 @@ struct H5 {
 };
 
 H5 g_h5;
+std::mutex g_h5_mu;   // load_h5 may be reached from several threads at once
 
 template <typename F>
 bool sym(void *so, const char *name, F &f)
@@ -78,6 +80,7 @@ bool sym(void *so, const char *name, F &f)
 
 int load_h5()
 {
+    std::lock_guard<std::mutex> lock(g_h5_mu);
     if (g_h5.so) return IEMIC_OK;
     const char *cands[3] = {std::getenv("IEMIC_HDF5_LIB"), "libhdf5.so", "/opt/conda/lib/libhdf5.so"};
     void *so = nullptr;
@@ -171,8 +174,11 @@ extern "C" int iemic_read_state(const char *path, double *x, int64_t n, double *
         rc = set_error(IEMIC_EIO, "iemic_read_state: cannot read State/Values");
     }
     if (rc == IEMIC_OK && (par || found)) {
+        // Model::loadStateFromFile stops with an error when the group is
+        // absent (Model.C:87-90); a missing single parameter is skipped
         const bool has = h.lexists(f, "Parameters", kDefault) > 0;
-        for (int i = 1; i <= 30; i++) {
+        if (!has) rc = set_error(IEMIC_EIO, std::string("iemic_read_state: no group <Parameters> in ") + path);
+        for (int i = 1; has && i <= 30; i++) {
             double v = 0.0;
             const std::string nm = std::string("Parameters/") + kLabels[i];
             const bool ok = has && read_all(f, nm.c_str(), *h.native_double, &v, 1, nullptr);

@@ -211,15 +211,18 @@ def read_mask(path, n, m, l):
     return out
 
 
-def read_state(path, ndim):
+def read_state(path, ndim, parameters=True):
     """A state file in the reference's EpetraExt-HDF5 layout
     (iemic_read_state, after Model::loadStateFromFile): returns (x, pars)
-    with pars {par.F90 index: value} for the parameters the file holds."""
+    with pars {par.F90 index: value} for the parameters the file holds (the
+    file must then have a /Parameters group, as the reference requires);
+    parameters=False reads the state alone."""
     x = np.zeros(ndim)
     par = np.zeros(31)
     found = np.zeros(31, np.int32)
-    _check(load().iemic_read_state(str(path).encode(), x.ctypes.data_as(_P(C.c_double)), ndim,
-                                   par.ctypes.data_as(_P(C.c_double)), found.ctypes.data_as(_P(C.c_int))))
+    pp = par.ctypes.data_as(_P(C.c_double)) if parameters else None
+    fp = found.ctypes.data_as(_P(C.c_int)) if parameters else None
+    _check(load().iemic_read_state(str(path).encode(), x.ctypes.data_as(_P(C.c_double)), ndim, pp, fp))
     return x, {i: float(par[i]) for i in range(1, 31) if found[i]}
 
 

@@ -44,11 +44,25 @@ def test_write_then_read_is_exact(tmp_path):
     x2, p2 = iemic_amd.read_state(p, NDIM)
     np.testing.assert_array_equal(x2, x)
     assert p2 == pars
-    # without parameters: the state alone, every parameter reported absent
+    # without parameters: the state alone; asking for parameters is then an
+    # error, as Model::loadStateFromFile stops without a <Parameters> group
     iemic_amd.write_state(p, x)
-    x3, p3 = iemic_amd.read_state(p, NDIM)
+    x3, p3 = iemic_amd.read_state(p, NDIM, parameters=False)
     np.testing.assert_array_equal(x3, x)
     assert p3 == {}
+    with pytest.raises(iemic_amd.IemicError) as e:
+        iemic_amd.read_state(p, NDIM)
+    assert e.value.code == -7 and "Parameters" in str(e.value)
+
+
+def test_concurrent_first_loads(tmp_path):
+    """Reads from several threads at once (in a fresh process the first of
+    them race to load the HDF5 library, which is loaded once)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(8) as ex:
+        outs = list(ex.map(lambda _: iemic_amd.read_state(H5, NDIM)[0], range(16)))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
 
 
 def test_errors(tmp_path):
