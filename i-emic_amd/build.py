@@ -22,7 +22,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-Wno-unused-variable"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
-SOURCES = ["host_setup.cpp", "capi.cpp", "assembly.hip", "precond.hip", "rowlu.hip", "dense_lu.hip", "dense_mv.hip", "solver.hip", "continuation.hip", "state_io.cpp", "noalloc.hip", "bilu.hip"]
+SOURCES = ["host_setup.cpp", "capi.cpp", "assembly.hip", "precond.hip", "rowlu.hip", "dense_lu.hip", "dense_mv.hip", "solver.hip", "continuation.hip", "state_io.cpp", "noalloc.hip", "bilu.hip", "block_gs.hip"]
 LIBS = ["-lrocsolver", "-lrocblas", "-pthread", "-ldl"]
 
 

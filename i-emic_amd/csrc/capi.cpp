@@ -24,6 +24,8 @@ int precond_info(iemic_ctx *c, iemic_precond_info *info);
 int krylov_solve(iemic_ctx *c, const iemic_solver_params *p, const double *d_b, double *d_x,
                  iemic_solve_info *info, bool transpose);
 int precond_apply_t(iemic_ctx *c, int precision, const double *in, double *out);
+int krylov_solve_block(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
+                       iemic_solve_info *infos, bool transpose);
 int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
                        iemic_solve_info *infos, bool transpose);
 int precond_apply_multi(iemic_ctx *c, int precision, int nrhs, const double *in, size_t ldin, double *out,
@@ -587,8 +589,31 @@ int iemic_solve_multi_transpose_dev(iemic_ctx *c, const iemic_solver_params *p, 
     return krylov_solve_multi(c, p ? p : &def, nrhs, d_B, d_X, infos, true);
 }
 
+int iemic_solve_block_dev(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *d_B, double *d_X,
+                          iemic_solve_info *infos)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    if (nrhs < 1 || nrhs > 32 || !d_B || !d_X) return set_error(IEMIC_EINVAL, "bad right-hand side block (1 to 32 columns)");
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return krylov_solve_block(c, p ? p : &def, nrhs, d_B, d_X, infos, false);
+}
+
+int iemic_solve_block_transpose_dev(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *d_B,
+                                    double *d_X, iemic_solve_info *infos)
+{
+    CHECK_CTX(c);
+    if (!c->have_jac) return set_error(IEMIC_ESTATE, "no Jacobian computed yet");
+    if (nrhs < 1 || nrhs > 32 || !d_B || !d_X) return set_error(IEMIC_EINVAL, "bad right-hand side block (1 to 32 columns)");
+    iemic_solver_params def;
+    iemic_solver_defaults(&def);
+    return krylov_solve_block(c, p ? p : &def, nrhs, d_B, d_X, infos, true);
+}
+
+// host copies around the device solves; block selects block FGMRES
 static int solve_multi_host(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
-                            iemic_solve_info *infos, bool transpose)
+                            iemic_solve_info *infos, bool transpose, bool block = false)
 {
     CHECK_CTX(c);
     if (nrhs < 1 || !B || !X) return set_error(IEMIC_EINVAL, "bad right-hand side block");
@@ -601,9 +626,14 @@ static int solve_multi_host(iemic_ctx *c, const iemic_solver_params *p, int nrhs
     }
     int rc = IEMIC_OK;
     if (hipMemcpy(dB, B, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy B");
-    if (!rc)
-        rc = transpose ? iemic_solve_multi_transpose_dev(c, p, nrhs, dB, dX, infos)
-                       : iemic_solve_multi_dev(c, p, nrhs, dB, dX, infos);
+    if (!rc) {
+        if (block)
+            rc = transpose ? iemic_solve_block_transpose_dev(c, p, nrhs, dB, dX, infos)
+                           : iemic_solve_block_dev(c, p, nrhs, dB, dX, infos);
+        else
+            rc = transpose ? iemic_solve_multi_transpose_dev(c, p, nrhs, dB, dX, infos)
+                           : iemic_solve_multi_dev(c, p, nrhs, dB, dX, infos);
+    }
     if (rc == IEMIC_OK || rc == IEMIC_ENOCONV) {
         const int keep = rc;
         if (hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = set_error(IEMIC_EHIP, "copy X");
@@ -624,6 +654,18 @@ int iemic_solve_multi_transpose(iemic_ctx *c, const iemic_solver_params *p, int 
                                 iemic_solve_info *infos)
 {
     return solve_multi_host(c, p, nrhs, B, X, infos, true);
+}
+
+int iemic_solve_block(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                      iemic_solve_info *infos)
+{
+    return solve_multi_host(c, p, nrhs, B, X, infos, false, true);
+}
+
+int iemic_solve_block_transpose(iemic_ctx *c, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                                iemic_solve_info *infos)
+{
+    return solve_multi_host(c, p, nrhs, B, X, infos, true, true);
 }
 
 int iemic_apply_precon_multi_dev(iemic_ctx *c, int nrhs, const double *d_in, double *d_out)

@@ -57,6 +57,10 @@ struct Krylov {
     // of every column, so column q's basis is a strided sequence
     double *mV = nullptr, *mZ = nullptr, *mW = nullptr, *mB = nullptr, *mh = nullptr;
     int mm = 0, mq = 0;
+    // block FGMRES: tall-skinny Gram partials, a Gram result and the small
+    // coefficient matrices of the block updates
+    double *bwork = nullptr, *bG = nullptr, *bH = nullptr;
+    size_t bwork_elems = 0, bsmall_elems = 0;
 };
 
 // ------------------------------------------------------------------ kernels
@@ -155,7 +159,7 @@ void krylov_free(iemic_ctx *c)
     Krylov *k = c->kry;
     if (!k) return;
     void *bufs[] = {k->b64.V, k->b64.Z, k->b64.w, k->d_h, k->d_part, k->d_r, k->d_bp, k->tmp32,
-                    k->mV, k->mZ, k->mW, k->mB, k->mh};
+                    k->mV, k->mZ, k->mW, k->mB, k->mh, k->bwork, k->bG, k->bH};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete k;
@@ -630,6 +634,514 @@ int krylov_solve_multi(iemic_ctx *c, const iemic_solver_params *p, int q, const 
         // factors from an older Jacobian missed the tolerance: refactor once
         if ((rc = precond_build(c, p->precision))) return rc;
         return krylov_solve_multi(c, p, q, d_B, d_X, infos, transpose);
+    }
+    all = true;
+    for (int r = 0; r < q; r++) {
+        iemic_solve_info inf{};
+        inf.iterations = its[r];
+        inf.restarts = cycles[r];
+        inf.rel_residual = bnorm[r] > 0 ? rn[r] / bnorm[r] : 0.0;
+        inf.t_precon_setup = std::chrono::duration<double>(t1 - t0).count();
+        inf.t_solve = t_solve;
+        inf.converged = inf.rel_residual <= p->tol;
+        all = all && inf.converged;
+        if (infos) infos[r] = inf;
+    }
+    return all ? IEMIC_OK : set_error(IEMIC_ENOCONV, "solver did not reach the tolerance for every right-hand side");
+}
+
+// ------------------------------------------------------------------ block FGMRES
+size_t ts_gram_work(int n, int a, int b);
+hipError_t ts_gram(hipStream_t st, int n, int a, int b, const double *A, size_t lda, const double *B, size_t ldb,
+                   double *G, int ldg, double *work);
+hipError_t ts_update(hipStream_t st, int n, int a, int b, double alpha, const double *A, size_t lda, const double *H,
+                     int ldh, double beta, double *C, size_t ldc);
+
+namespace {
+
+// S = Q diag(w) Q^T for the symmetric p x p matrix S (column-major), by
+// cyclic Jacobi rotations (Golub and Van Loan, Matrix Computations, 8.5);
+// p is at most the block size, so this costs nothing next to one pass
+// over the basis.
+void sym_eig(int p, std::vector<double> S, std::vector<double> &w, std::vector<double> &Q)
+{
+    Q.assign((size_t)p * p, 0.0);
+    for (int i = 0; i < p; i++) Q[i + (size_t)i * p] = 1.0;
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int j = 0; j < p; j++)
+            for (int i = 0; i < p; i++) {
+                const double v = S[i + (size_t)j * p] * S[i + (size_t)j * p];
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (!(off > 1e-32 * tot)) break;
+        for (int i = 0; i < p - 1; i++)
+            for (int j = i + 1; j < p; j++) {
+                const double aij = S[i + (size_t)j * p];
+                if (aij == 0.0) continue;
+                const double theta = (S[j + (size_t)j * p] - S[i + (size_t)i * p]) / (2.0 * aij);
+                const double t = std::copysign(1.0, theta) / (std::fabs(theta) + std::hypot(theta, 1.0));
+                const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < p; k++) {   // columns i and j
+                    const double ski = S[k + (size_t)i * p], skj = S[k + (size_t)j * p];
+                    S[k + (size_t)i * p] = cs * ski - sn * skj;
+                    S[k + (size_t)j * p] = sn * ski + cs * skj;
+                }
+                for (int k = 0; k < p; k++) {   // rows i and j
+                    const double sik = S[i + (size_t)k * p], sjk = S[j + (size_t)k * p];
+                    S[i + (size_t)k * p] = cs * sik - sn * sjk;
+                    S[j + (size_t)k * p] = sn * sik + cs * sjk;
+                }
+                for (int k = 0; k < p; k++) {
+                    const double qki = Q[k + (size_t)i * p], qkj = Q[k + (size_t)j * p];
+                    Q[k + (size_t)i * p] = cs * qki - sn * qkj;
+                    Q[k + (size_t)j * p] = sn * qki + cs * qkj;
+                }
+            }
+    }
+    w.resize(p);
+    for (int i = 0; i < p; i++) w[i] = S[i + (size_t)i * p];
+}
+
+// G = R^T R, R upper triangular (p x p, column-major); false if G is not
+// numerically positive definite
+bool cholesky_upper(int p, const std::vector<double> &G, std::vector<double> &R)
+{
+    R.assign((size_t)p * p, 0.0);
+    for (int j = 0; j < p; j++) {
+        for (int i = 0; i <= j; i++) {
+            double s = G[i + (size_t)j * p];
+            for (int k = 0; k < i; k++) s -= R[k + (size_t)i * p] * R[k + (size_t)j * p];
+            if (i == j) {
+                if (!(s > 0.0)) return false;
+                R[j + (size_t)j * p] = std::sqrt(s);
+            } else {
+                R[i + (size_t)j * p] = s / R[i + (size_t)i * p];
+            }
+        }
+    }
+    return true;
+}
+
+// inverse of the upper triangular R (p x p, column-major)
+std::vector<double> upper_inverse(int p, const std::vector<double> &R)
+{
+    std::vector<double> X((size_t)p * p, 0.0);
+    for (int j = 0; j < p; j++) {
+        X[j + (size_t)j * p] = 1.0 / R[j + (size_t)j * p];
+        for (int i = j - 1; i >= 0; i--) {
+            double s = 0.0;
+            for (int k = i + 1; k <= j; k++) s += R[i + (size_t)k * p] * X[k + (size_t)j * p];
+            X[i + (size_t)j * p] = -s / R[i + (size_t)i * p];
+        }
+    }
+    return X;
+}
+
+// Rank split of a block W (p columns) from its Gram matrix G = W^T W: the
+// eigen-decomposition G = Q diag(w) Q^T, the directions with w above
+// max(rel * largest, floor) kept.  Returns their number k and the p x p
+// rotation H whose first k columns are Q_k diag(w_k)^{-1/2} (W H then has
+// orthonormal leading columns, up to the rounding of G) and whose other
+// columns are the dropped eigenvectors (W H there holds the dropped part);
+// *dropped is an upper bound of the dropped singular values, including what
+// the rounding of G hides (eps of the largest eigenvalue).
+int rank_split(int p, const std::vector<double> &G, double rel, double floor, std::vector<double> &H, double *dropped)
+{
+    std::vector<double> w, Q;
+    sym_eig(p, G, w, Q);
+    std::vector<int> order(p);
+    for (int i = 0; i < p; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
+    const double wmax = std::max(0.0, w[order[0]]);
+    int k = 0;
+    double wd = 0.0;
+    for (int i = 0; i < p; i++) {
+        const double v = w[order[i]];
+        if (v > rel * wmax && v > floor) k++;
+        else wd = std::max(wd, v);
+    }
+    H.assign((size_t)p * p, 0.0);
+    for (int c = 0; c < p; c++) {
+        const int e = order[c];
+        const double sc = c < k ? 1.0 / std::sqrt(w[e]) : 1.0;
+        for (int i = 0; i < p; i++) H[i + (size_t)c * p] = Q[i + (size_t)e * p] * sc;
+    }
+    *dropped = k == p ? 0.0 : std::sqrt(std::max(wd, 0.0) + 1e-16 * wmax * p);
+    return k;
+}
+
+}  // namespace
+
+static int ensure_block(iemic_ctx *c, int m, int q)
+{
+    Krylov *k = c->kry;
+    const int capV = (m + 1) * q;
+    const size_t wk = ts_gram_work(c->sc.ndim, capV, q), sm = (size_t)capV * q;
+    if (k->bwork_elems < wk) {
+        if (k->bwork) (void)hipFree(k->bwork);
+        k->bwork = nullptr;
+        k->bwork_elems = 0;
+        HIPCHK(hipMalloc(&k->bwork, wk * sizeof(double)));
+        k->bwork_elems = wk;
+    }
+    if (k->bsmall_elems < sm) {
+        if (k->bG) (void)hipFree(k->bG);
+        if (k->bH) (void)hipFree(k->bH);
+        k->bG = k->bH = nullptr;
+        k->bsmall_elems = 0;
+        HIPCHK(hipMalloc(&k->bG, sm * sizeof(double)));
+        HIPCHK(hipMalloc(&k->bH, sm * sizeof(double)));
+        k->bsmall_elems = sm;
+    }
+    return IEMIC_OK;
+}
+
+// J X = B (or J^T X = B) for q <= 32 right-hand sides by block FGMRES: the
+// columns share one Krylov space, so a direction found for one column serves
+// all of them.  A cycle starts from the block of residuals of the columns
+// not yet converged; each iteration
+//   Z_j = M^{-1} V_j (one batched sweep), W = J Z_j (one SpMM),
+//   block classical Gram-Schmidt twice against [V_0 .. V_j] (W -= V (V^T W)),
+//   then W orthonormalised by the eigen-decomposition of its Gram matrix
+//   (SVQB, Stathopoulos and Wu 2002) and once more by Cholesky QR,
+// every tall-skinny product on the matrix cores (block_gs.hip).  Directions
+// whose singular value falls below 1e-7 of the largest (or below 1e-12 of
+// the block's scale) are dropped: they already lie in the space (the
+// preconditioner of this library differs from J by a few pinned rows, so
+// after one iteration the new block has only a few directions left, whatever
+// the number of right-hand sides).  The block upper-Hessenberg matrix is
+// reduced by Householder reflections as it grows; the least-squares
+// residual of each column decides when the cycle ends, the true FP64
+// residuals decide which columns restart.  Each column runs on the
+// consistent part of its right-hand side, as in krylov_solve.
+int krylov_solve_block(iemic_ctx *c, const iemic_solver_params *p, int q, const double *d_B, double *d_X,
+                       iemic_solve_info *infos, bool transpose)
+{
+    using clk = std::chrono::steady_clock;
+    if (q < 1 || p->restart < 1 || p->maxit < 1 || p->tol <= 0.0 || (p->precision != 0 && p->precision != 1))
+        return set_error(IEMIC_EINVAL, "solver parameters out of range");
+    if (q > 32) return set_error(IEMIC_EINVAL, "block solve: at most 32 right-hand sides per call");
+    const int m = p->restart;
+    int rc = ensure_krylov(c, m);
+    if (rc) return rc;
+    if ((rc = ensure_multi(c, m, q))) return rc;
+    if ((rc = ensure_block(c, m, q))) return rc;
+    Krylov *k = c->kry;
+    k->precision = p->precision;
+    k->transpose = transpose;
+    const int n = c->sc.ndim;
+    const size_t N = (size_t)n;
+    hipStream_t st = c->stream;
+    auto prec = [&](int nr, const double *in, double *out) {
+        return transpose ? precond_apply_multi_t(c, p->precision, nr, in, N, out, N)
+                         : precond_apply_multi(c, p->precision, nr, in, N, out, N, k->tmp32);
+    };
+    auto mat = [&](int nr, const double *in, double *out) {
+        return transpose ? launch_spmm_t(c, nr, in, N, out, N) : launch_spmm(c, nr, in, N, out, N);
+    };
+    // G = A^T Bm (a x b) on the host
+    auto gram = [&](int a, const double *A, int b, const double *Bm, std::vector<double> &out) -> int {
+        HIPCHK(ts_gram(st, n, a, b, A, N, Bm, N, k->bG, a, k->bwork));
+        out.resize((size_t)a * b);
+        HIPCHK(hipMemcpyAsync(out.data(), k->bG, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return IEMIC_OK;
+    };
+    // Cm = beta Cm + alpha A H (H: a x b on the host, leading dimension ldh)
+    auto update = [&](int a, const double *A, int b, const double *H, int ldh, double alpha, double beta,
+                      double *Cm) -> int {
+        if (b <= 0) return IEMIC_OK;
+        if (a > 0) HIPCHK(hipMemcpyAsync(k->bH, H, (size_t)ldh * b * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(ts_update(st, n, a, b, alpha, A, N, k->bH, ldh, beta, Cm, N));
+        // the host matrix may change as soon as this returns
+        HIPCHK(hipStreamSynchronize(st));
+        return IEMIC_OK;
+    };
+    const int nb = nblk(n, 256);
+    auto t0 = clk::now();
+    if ((rc = precond_ready(c, p->precision))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    auto t1 = clk::now();
+    // column norms of a block of at most 32 columns: the diagonal of its
+    // Gram matrix (one pass)
+    auto col_norms = [&](int nq, const double *P, std::vector<double> &out) -> int {
+        std::vector<double> gg;
+        if ((rc = gram(nq, P, nq, P, gg))) return rc;
+        out.resize(nq);
+        for (int i = 0; i < nq; i++) out[i] = std::sqrt(std::max(gg[i + (size_t)i * nq], 0.0));
+        return IEMIC_OK;
+    };
+    std::vector<double> bnorm, bcn;
+    if ((rc = col_norms(q, d_B, bnorm))) return rc;
+    // consistent right-hand sides: the components along the null vectors
+    // (mutually orthogonal: disjoint pressure basins) removed from all
+    // columns at once, B' = B - N diag(1 / |n_i|^2) N^T B
+    HIPCHK(hipMemcpyAsync(k->mB, d_B, (size_t)q * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    {
+        const double *vecs, *nrm2;
+        int cnt;
+        if ((rc = precond_null_space(c, p->precision, !transpose, &vecs, &nrm2, &cnt))) return rc;
+        const int capA = (m + 1) * q;
+        for (int i0 = 0; i0 < cnt; i0 += capA) {
+            const int ci = std::min(capA, cnt - i0);
+            const double *Vn = vecs + (size_t)i0 * N;
+            std::vector<double> gv;
+            if ((rc = gram(ci, Vn, q, k->mB, gv))) return rc;
+            for (int j2 = 0; j2 < q; j2++)
+                for (int i = 0; i < ci; i++) {
+                    const double nn = nrm2[i0 + i];
+                    gv[i + (size_t)j2 * ci] = nn > 0.0 ? gv[i + (size_t)j2 * ci] / nn : 0.0;
+                }
+            if ((rc = update(ci, Vn, q, gv.data(), ci, -1.0, 1.0, k->mB))) return rc;
+        }
+    }
+    if ((rc = col_norms(q, k->mB, bcn))) return rc;
+    for (int r = 0; r < q; r++) hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, st, d_X + r * N, n);
+    double *R = k->mW;
+    HIPCHK(hipMemcpyAsync(R, k->mB, (size_t)q * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    std::vector<int> its(q, 0), cycles(q, 0);
+    std::vector<char> done(q, 0);
+    for (int r = 0; r < q; r++) done[r] = !(bcn[r] > 0.0);
+    const int capV = (m + 1) * q, capZ = m * q;
+    std::vector<double> Hq((size_t)capV * capZ), Gr, Hcol, tau, beta = bcn;
+    std::vector<std::vector<double>> refl;
+    double *V = k->mV, *Z = k->mZ;
+    // Orthonormalise the pc columns at V + nbas N against V[0, nbas) and
+    // among themselves: block CGS twice, then the rank split of what is
+    // left, the kept directions first and orthonormalised once more by
+    // Cholesky QR, the dropped remainder after them.  sref (< 0: measured
+    // here, from the block before projection) scales the absolute floor
+    // tau_fac * sref below which directions are dropped.
+    auto ortho_block = [&](int nbas, int pc, double tau_fac, double &sref, int &kept, double &dropped) -> int {
+        double *W = V + (size_t)nbas * N;
+        const int lg = nbas + pc;
+        std::vector<double> g1, g2, Gw((size_t)pc * pc), Hr, M;
+        if ((rc = gram(lg, V, pc, W, g1))) return rc;
+        if (sref < 0.0) {
+            sref = 0.0;
+            for (int i = 0; i < pc; i++) sref = std::max(sref, g1[nbas + i + (size_t)i * lg]);
+            sref = std::sqrt(sref);
+        }
+        if (nbas > 0) {
+            // W <- W - V C1 = [V W] [-C1; I] (W is the tail of [V W])
+            M.assign((size_t)lg * pc, 0.0);
+            for (int j2 = 0; j2 < pc; j2++) {
+                for (int t = 0; t < nbas; t++) M[t + (size_t)j2 * lg] = -g1[t + (size_t)j2 * lg];
+                M[nbas + j2 + (size_t)j2 * lg] = 1.0;
+            }
+            if ((rc = update(lg, V, pc, M.data(), lg, 1.0, 0.0, W))) return rc;
+            if ((rc = gram(lg, V, pc, W, g2))) return rc;
+        } else {
+            g2 = g1;
+        }
+        for (int b2 = 0; b2 < pc; b2++)
+            for (int a2 = 0; a2 < pc; a2++) {
+                double sum = g2[nbas + a2 + (size_t)b2 * lg];
+                for (int t = 0; t < nbas; t++) sum -= g2[t + (size_t)a2 * lg] * g2[t + (size_t)b2 * lg];
+                Gw[a2 + (size_t)b2 * pc] = sum;
+            }
+        const double fl = tau_fac * sref;
+        int kk = rank_split(pc, Gw, 1e-14, fl * fl, Hr, &dropped);
+        // second projection and the rank split's rotation in one pass:
+        // W <- (W - V C2) Hr = [V W] [-C2 Hr; Hr]
+        M.assign((size_t)lg * pc, 0.0);
+        for (int j2 = 0; j2 < pc; j2++) {
+            for (int t = 0; t < nbas; t++) {
+                double sum = 0.0;
+                for (int u = 0; u < pc; u++) sum += g2[t + (size_t)u * lg] * Hr[u + (size_t)j2 * pc];
+                M[t + (size_t)j2 * lg] = -sum;
+            }
+            for (int i = 0; i < pc; i++) M[nbas + i + (size_t)j2 * lg] = Hr[i + (size_t)j2 * pc];
+        }
+        if ((rc = update(lg, V, pc, M.data(), lg, 1.0, 0.0, W))) return rc;
+        if (kk > 0) {
+            // Cholesky QR of the kept columns, with their last projection:
+            // W_k <- (W_k - V C3) R3^{-1} = [V W_k] [-C3 R3^{-1}; R3^{-1}]
+            std::vector<double> g3, G3((size_t)kk * kk), R3;
+            const int l3 = nbas + kk;
+            if ((rc = gram(l3, V, kk, W, g3))) return rc;
+            for (int b2 = 0; b2 < kk; b2++)
+                for (int a2 = 0; a2 < kk; a2++) {
+                    double sum = g3[nbas + a2 + (size_t)b2 * l3];
+                    for (int t = 0; t < nbas; t++) sum -= g3[t + (size_t)a2 * l3] * g3[t + (size_t)b2 * l3];
+                    G3[a2 + (size_t)b2 * kk] = sum;
+                }
+            if (!cholesky_upper(kk, G3, R3)) {
+                kk = 0;   // lost to rounding: drop the whole block
+                dropped = sref;
+            } else {
+                std::vector<double> Ri = upper_inverse(kk, R3);
+                M.assign((size_t)l3 * kk, 0.0);
+                for (int j2 = 0; j2 < kk; j2++) {
+                    for (int t = 0; t < nbas; t++) {
+                        double sum = 0.0;
+                        for (int u = 0; u <= j2; u++) sum += g3[t + (size_t)u * l3] * Ri[u + (size_t)j2 * kk];
+                        M[t + (size_t)j2 * l3] = -sum;
+                    }
+                    for (int i = 0; i <= j2; i++) M[nbas + i + (size_t)j2 * l3] = Ri[i + (size_t)j2 * kk];
+                }
+                if ((rc = update(l3, V, kk, M.data(), l3, 1.0, 0.0, W))) return rc;
+            }
+        }
+        kept = kk;
+        return IEMIC_OK;
+    };
+    // up to three rank splits, each resolving the previous one's dropped
+    // remainder relative to its own scale, until what is dropped lies below
+    // the absolute floor; returns the number of orthonormal columns added
+    auto extend_basis = [&](int nbas, int pc, double tau_fac, int &added) -> int {
+        double sref = -1.0;
+        added = 0;
+        for (int lev = 0; lev < 3 && added < pc; lev++) {
+            int kk;
+            double dropped;
+            if ((rc = ortho_block(nbas + added, pc - added, tau_fac, sref, kk, dropped))) return rc;
+            added += kk;
+            if (kk == 0 || dropped <= tau_fac * sref) break;
+        }
+        return IEMIC_OK;
+    };
+    const double tau_fac = 0.1 * p->tol;
+    auto ms = [](clk::time_point a, clk::time_point b) { return 1e3 * std::chrono::duration<double>(b - a).count(); };
+    while (true) {
+        std::vector<int> live;
+        for (int r = 0; r < q; r++)
+            if (!done[r] && its[r] < p->maxit) live.push_back(r);
+        const int ql = (int)live.size();
+        if (ql == 0) break;
+        // V_0 S_0 = the residuals of the live columns: moved to the front of
+        // R (live[i] >= i, so no column is overwritten before it is read),
+        // scaled to unit norm into V, orthonormalised, S_0 = V_0^T R
+        for (int i = 0; i < ql; i++) {
+            if (live[i] != i)
+                HIPCHK(hipMemcpyAsync(R + i * N, R + live[i] * N, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_scale_copy<double>, dim3(nb), dim3(256), 0, st, V + i * N, R + i * N,
+                               1.0 / beta[live[i]], n);
+        }
+        int p0 = 0;
+        if ((rc = extend_basis(0, ql, tau_fac, p0))) return rc;
+        if (p0 == 0) break;
+        std::vector<double> s0;
+        if ((rc = gram(p0, V, ql, R, s0))) return rc;
+        Gr.assign((size_t)capV * ql, 0.0);
+        for (int cq = 0; cq < ql; cq++)
+            for (int i = 0; i < p0; i++) Gr[i + (size_t)cq * capV] = s0[i + (size_t)cq * p0];
+        int nV = p0, nZ = 0, off = 0, pj = p0;
+        refl.clear();
+        tau.clear();
+        std::fill(Hq.begin(), Hq.end(), 0.0);
+        for (int j = 0; j < m; j++) {
+            const int pin = pj;
+            double *Vj = V + (size_t)off * N, *Zj = Z + (size_t)off * N, *W = V + (size_t)nV * N;
+            // Z_j = M^{-1} V_j; J Z_j kept in R (free until the restart) and
+            // copied to the next block's place in V
+            auto ta = clk::now();
+            if ((rc = prec(pj, Vj, Zj))) return rc;
+            if (p->verbose) HIPCHK(hipStreamSynchronize(st));
+            auto tb = clk::now();
+            if ((rc = mat(pj, Zj, R))) return rc;
+            HIPCHK(hipMemcpyAsync(W, R, (size_t)pj * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+            if (p->verbose) HIPCHK(hipStreamSynchronize(st));
+            auto tc = clk::now();
+            int p1 = 0;
+            if ((rc = extend_basis(nV, pj, tau_fac, p1))) return rc;
+            if (nV + p1 > capV) p1 = std::max(0, capV - nV);
+            // H block column: [V_0 .. V_j+1]^T J Z_j
+            const int L = nV + p1;
+            if ((rc = gram(L, V, pj, R, Hcol))) return rc;
+            auto td = clk::now();
+            // Householder QR of the grown Hessenberg matrix, applied to Gr
+            for (int i = 0; i < pj; i++) {
+                const int col = nZ + i;
+                double *h = Hq.data() + (size_t)col * capV;
+                for (int t = 0; t < L; t++) h[t] = Hcol[t + (size_t)i * L];
+                for (int kq = 0; kq < col; kq++) {
+                    const std::vector<double> &v = refl[kq];
+                    double w = 0.0;
+                    for (size_t t = 0; t < v.size(); t++) w += v[t] * h[kq + t];
+                    w *= tau[kq];
+                    for (size_t t = 0; t < v.size(); t++) h[kq + t] -= w * v[t];
+                }
+                std::vector<double> v(L - col, 0.0);
+                const double alpha = h[col];
+                double xn = 0.0;
+                for (int t = col + 1; t < L; t++) xn = std::hypot(xn, h[t]);
+                double tk = 0.0, bet = alpha;
+                v[0] = 1.0;
+                if (xn > 0.0) {
+                    bet = -std::copysign(std::hypot(alpha, xn), alpha);
+                    tk = (bet - alpha) / bet;
+                    for (int t = col + 1; t < L; t++) v[t - col] = h[t] / (alpha - bet);
+                }
+                h[col] = bet;
+                for (int t = col + 1; t < L; t++) h[t] = 0.0;
+                for (int cq = 0; cq < ql; cq++) {
+                    double *gcol = Gr.data() + (size_t)cq * capV;
+                    double w = 0.0;
+                    for (int t = col; t < L; t++) w += v[t - col] * gcol[t];
+                    w *= tk;
+                    for (int t = col; t < L; t++) gcol[t] -= w * v[t - col];
+                }
+                refl.push_back(std::move(v));
+                tau.push_back(tk);
+            }
+            nZ += pj;
+            off = nV;
+            nV = L;
+            pj = p1;
+            bool conv = true;
+            for (int cq = 0; cq < ql; cq++) {
+                its[live[cq]]++;
+                double e = 0.0;
+                for (int t = nZ; t < nV; t++) e = std::hypot(e, Gr[t + (size_t)cq * capV]);
+                if (!(e < p->tol * bcn[live[cq]])) conv = false;
+            }
+            if (p->verbose)
+                std::fprintf(stderr,
+                             "block FGMRES: iteration %d, block %d -> %d directions, basis %d; ms: preconditioner "
+                             "%.2f, J %.2f, orthonormalisation %.2f\n",
+                             j + 1, pin, p1, nV, ms(ta, tb), ms(tb, tc), ms(tc, td));
+            if (conv || p1 == 0 || its[live[0]] >= p->maxit || nZ + p1 > capZ) break;
+        }
+        // Y = R^{-1} Gr (least squares), X += Z Y
+        std::vector<double> Y((size_t)nZ * q, 0.0);
+        double dmax = 0.0;
+        for (int i = 0; i < nZ; i++) dmax = std::max(dmax, std::fabs(Hq[i + (size_t)i * capV]));
+        for (int cq = 0; cq < ql; cq++) {
+            double *y = Y.data() + (size_t)live[cq] * nZ;
+            for (int i = nZ - 1; i >= 0; i--) {
+                double s = Gr[i + (size_t)cq * capV];
+                for (int u = i + 1; u < nZ; u++) s -= Hq[i + (size_t)u * capV] * y[u];
+                const double d = Hq[i + (size_t)i * capV];
+                y[i] = std::fabs(d) > 1e-14 * dmax ? s / d : 0.0;
+            }
+        }
+        if ((rc = update(nZ, Z, q, Y.data(), nZ, 1.0, 1.0, d_X))) return rc;
+        for (int r : live) cycles[r]++;
+        // true residuals R = B' - J X of the consistent systems
+        if ((rc = mat(q, d_X, R))) return rc;
+        for (int r = 0; r < q; r++)
+            hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, st, R + r * N, k->mB + r * N, 1.0, -1.0, n);
+        if ((rc = col_norms(q, R, beta))) return rc;
+        for (int r : live)
+            if (beta[r] / bcn[r] < p->tol) done[r] = 1;
+    }
+    // reported residuals against the original right-hand sides
+    if ((rc = mat(q, d_X, R))) return rc;
+    for (int r = 0; r < q; r++)
+        hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, st, R + r * N, d_B + r * N, 1.0, -1.0, n);
+    std::vector<double> rn;
+    if ((rc = col_norms(q, R, rn))) return rc;
+    const double t_solve = std::chrono::duration<double>(clk::now() - t1).count();
+    if ((rc = precond_check(c))) return rc;
+    bool all = true;
+    for (int r = 0; r < q; r++)
+        if (bnorm[r] > 0 && !(rn[r] / bnorm[r] <= p->tol)) all = false;
+    if (!all && precond_stale(c, p->precision)) {
+        if ((rc = precond_build(c, p->precision))) return rc;
+        return krylov_solve_block(c, p, q, d_B, d_X, infos, transpose);
     }
     all = true;
     for (int r = 0; r < q; r++) {

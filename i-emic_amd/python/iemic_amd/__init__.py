@@ -130,6 +130,12 @@ def load():
         "iemic_solve_multi_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
         "iemic_solve_multi_transpose": ([vp, vp, C.c_int, dp, dp, vp], C.c_int),
         "iemic_solve_multi_transpose_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
+        "iemic_solve_block": ([vp, vp, C.c_int, dp, dp, vp], C.c_int),
+        "iemic_solve_block_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
+        "iemic_solve_block_transpose": ([vp, vp, C.c_int, dp, dp, vp], C.c_int),
+        "iemic_solve_block_transpose_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
+        "iemic_block_gram_dev": ([C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
+        "iemic_block_update_dev": ([C.c_int, C.c_int, C.c_int, C.c_double, vp, vp, C.c_double, vp], C.c_int),
         "iemic_apply_precon_multi_dev": ([vp, C.c_int, vp, vp], C.c_int),
         "iemic_newton_dev": ([vp, vp, C.c_double, C.c_double, C.c_int, vp, ip, dp], C.c_int),
         "iemic_continuation_defaults": ([vp], None),
@@ -630,9 +636,11 @@ class Ocean:
         k = info.steps + 1
         return dx.download(), ph[:k], nh[:k], info
 
-    def solveMulti(self, B, params=None, raise_on_noconv=True, transpose=False):
+    def solveMulti(self, B, params=None, raise_on_noconv=True, transpose=False, block=False):
         """J X = B (J^T X = B if transpose) for the columns of B (ndim x
-        nrhs) in one batched solve; returns (X, [SolveInfo per column])."""
+        nrhs) in one batched solve; returns (X, [SolveInfo per column]).
+        block: block FGMRES over one shared Krylov space (at most 32
+        columns) instead of one FGMRES per column in lock step."""
         B = np.asarray(B, dtype=np.float64)
         if B.ndim == 1:
             B = B[:, None]
@@ -641,18 +649,24 @@ class Ocean:
         Xc = np.zeros_like(Bc)
         infos = (SolveInfo * nrhs)()
         p = params if params is not None else self.solver_params()
-        fn = self.lib.iemic_solve_multi_transpose if transpose else self.lib.iemic_solve_multi
+        if block:
+            fn = self.lib.iemic_solve_block_transpose if transpose else self.lib.iemic_solve_block
+        else:
+            fn = self.lib.iemic_solve_multi_transpose if transpose else self.lib.iemic_solve_multi
         rc = fn(self.h, C.byref(p), nrhs, _d(Bc), _d(Xc), infos)
         if rc != 0 and (raise_on_noconv or rc != -6):
             _check(rc)
         return Xc.T.copy(), list(infos)
 
-    def solve_multi_dev(self, dB, dX, nrhs, params=None, raise_on_noconv=True, transpose=False):
+    def solve_multi_dev(self, dB, dX, nrhs, params=None, raise_on_noconv=True, transpose=False, block=False):
         """J X = B (J^T X = B if transpose) for nrhs columns stored one after
-        another in device arrays."""
+        another in device arrays (block: block FGMRES, at most 32 columns)."""
         infos = (SolveInfo * nrhs)()
         p = params if params is not None else self.solver_params()
-        fn = self.lib.iemic_solve_multi_transpose_dev if transpose else self.lib.iemic_solve_multi_dev
+        if block:
+            fn = self.lib.iemic_solve_block_transpose_dev if transpose else self.lib.iemic_solve_block_dev
+        else:
+            fn = self.lib.iemic_solve_multi_transpose_dev if transpose else self.lib.iemic_solve_multi_dev
         rc = fn(self.h, C.byref(p), nrhs, dB.ptr, dX.ptr, infos)
         if rc != 0 and (raise_on_noconv or rc != -6):
             _check(rc)

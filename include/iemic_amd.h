@@ -249,6 +249,31 @@ int iemic_solve_multi_transpose(iemic_ctx *ctx, const iemic_solver_params *p, in
                                 iemic_solve_info *infos);
 int iemic_solve_multi_transpose_dev(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *d_B,
                                     double *d_X, iemic_solve_info *infos);
+/* J X = B (and J^T X = B) for 1 to 32 columns by block FGMRES: one Krylov
+ * space shared by all columns (the Belos block GMRES that Ocean::solve's
+ * solver manager can run on a multivector, generalising the Gram-Schmidt
+ * loop of GMRESSolver.H:177-181 to blocks).  The block Gram-Schmidt and
+ * basis updates run on the matrix cores; directions already in the space
+ * are deflated, so after the first iteration the block shrinks to the few
+ * directions the preconditioner misses.  restart = block iterations per
+ * cycle; iterations in infos count the block iterations each column took
+ * part in.  Same contract as iemic_solve_multi otherwise. */
+int iemic_solve_block(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                      iemic_solve_info *infos);
+int iemic_solve_block_dev(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *d_B,
+                          double *d_X, iemic_solve_info *infos);
+int iemic_solve_block_transpose(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *B, double *X,
+                                iemic_solve_info *infos);
+int iemic_solve_block_transpose_dev(iemic_ctx *ctx, const iemic_solver_params *p, int nrhs, const double *d_B,
+                                    double *d_X, iemic_solve_info *infos);
+/* the tall-skinny products of the block solver on their own (no counterpart
+ * in the reference; exposed so that they can be checked against a host
+ * product): G = A^T B (a x b) and C = beta C + alpha A H (H: a x b), column-
+ * major operands with leading dimension n (G and H: a), b <= 32; C may be A
+ * itself when b <= a.  Null stream; returns when done. */
+int iemic_block_gram_dev(int n, int a, int b, const double *d_A, const double *d_B, double *d_G);
+int iemic_block_update_dev(int n, int a, int b, double alpha, const double *d_A, const double *d_H,
+                           double beta, double *d_C);
 /* out = M^{-1} in on nrhs columns (FP64 factors) */
 int iemic_apply_precon_multi_dev(iemic_ctx *ctx, int nrhs, const double *d_in, double *d_out);
 

@@ -5,7 +5,7 @@ on the host from the exported Jacobian:
 * the 180x80x24 seeded global basin: one Newton step (F, J, preconditioner,
   FGMRES to 1e-8), and the 32-column batch (24 forward columns J y, y ~
   N(0,1), and 8 adjoint columns: the two steady-state diagnostics' gradients
-  and six J^T y);
+  and six J^T y), by the lock-step solver and by block FGMRES;
 * the eight mixed-size seeded basins, one Newton step each, on concurrent
   host threads and streams and one after the other;
 * the 96x38x12 seeded basin: pseudo-arclength continuation in COMB from
@@ -84,6 +84,30 @@ def test_batch_of_32_at_180x80x24(basin180):
     L, infos = oc.solveMulti(G, p, transpose=True)
     rels = [np.linalg.norm(G[:, q] - J.T @ L[:, q]) / np.linalg.norm(G[:, q]) for q in range(8)]
     print(f"8 adjoint columns: iterations {[i.iterations for i in infos]}, max residual {max(rels):.2e}")
+    assert max(rels) <= 1e-8
+
+
+def test_block_batch_of_32_at_180x80x24(basin180):
+    """The same 24 forward and 8 adjoint columns by block FGMRES (one Krylov
+    space per batch, block Gram-Schmidt on the matrix cores), each column to
+    the FP64 bar."""
+    oc, x0 = basin180
+    oc.computeJacobian(x0)
+    J = _csr(oc)
+    rng = np.random.default_rng(13)
+    p = oc.solver_params(tol=1e-8, restart=10, maxit=100)
+    B = np.stack([J @ rng.normal(size=oc.ndim) for _ in range(24)], axis=1)
+    X, infos = oc.solveMulti(B, p, block=True)
+    rels = [np.linalg.norm(B[:, q] - J @ X[:, q]) / np.linalg.norm(B[:, q]) for q in range(24)]
+    print(f"block, 24 forward columns: iterations {[i.iterations for i in infos]}, max residual {max(rels):.2e}")
+    assert max(rels) <= 1e-8
+    del B, X
+    G = [oc.diagnostic(iemic_amd.DIAG_MEAN_SST, x0)[1], oc.diagnostic(iemic_amd.DIAG_MAX_PSIM, x0)[1]]
+    G += [J.T @ rng.normal(size=oc.ndim) for _ in range(6)]
+    G = np.stack(G, axis=1)
+    L, infos = oc.solveMulti(G, p, transpose=True, block=True)
+    rels = [np.linalg.norm(G[:, q] - J.T @ L[:, q]) / np.linalg.norm(G[:, q]) for q in range(8)]
+    print(f"block, 8 adjoint columns: iterations {[i.iterations for i in infos]}, max residual {max(rels):.2e}")
     assert max(rels) <= 1e-8
 
 
