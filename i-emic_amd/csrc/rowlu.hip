@@ -42,16 +42,11 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "dense_lu.hpp"
 
 namespace iemic {
 
 std::vector<int> column_tops(const iemic_ctx *c);
-template <typename T>
-int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *info, void *work, int *perm);
-template <typename T>
-int dense_inverse(rocblas_handle h, hipStream_t s, int n, T *LU, const int *perm, T *W);
-size_t dense_getrf_work_bytes(int n);
-void dense_getrf_size_query(rocblas_handle h, int n);
 template <typename T>
 hipError_t dense_mm(hipStream_t st, int m, int k, int q, T alpha, const T *A, size_t lda, const T *X, size_t ldx,
                     T beta, T *Y, size_t ldy, T *work, size_t work_elems, int blocks);
@@ -95,6 +90,8 @@ struct RowLU {
     int mcap = 0;
     void *d_w = nullptr;                  // nmax^2 scratch: right-hand sides of the inversion
     rocblas_handle blas = nullptr;
+    LuAhead ahead;                        // look-ahead stream, handle and events of dense_getrf
+    bool lookahead = false;               // true (IEMIC_LU_LOOKAHEAD=1): panels one block ahead on a second stream
     // null vectors of J, one per pressure basin b (pinned row r_b):
     // left m_b = M^{-T} e_{r_b}, right n_b = M^{-1} e_{r_b}
     std::vector<int> pin_rows;
@@ -375,6 +372,10 @@ void rowlu_free(iemic_ctx *c)
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (p->blas) rocblas_destroy_handle(p->blas);
+    if (p->ahead.h2) rocblas_destroy_handle(p->ahead.h2);
+    if (p->ahead.ev_panel) (void)hipEventDestroy(p->ahead.ev_panel);
+    if (p->ahead.ev_rest) (void)hipEventDestroy(p->ahead.ev_rest);
+    if (p->ahead.s2) (void)hipStreamDestroy(p->ahead.s2);
     delete p;
     c->rowlu = nullptr;
 }
@@ -524,6 +525,21 @@ static int rowlu_ensure(iemic_ctx *c)
         dense_getrf_size_query(p->blas, p->nmax);
         BLASCHK(rocblas_stop_device_memory_size_query(p->blas, &ws));
         BLASCHK(rocblas_set_device_memory_size(p->blas, ws));
+        // the look-ahead stream of the dense LU: its own handle, sized the
+        // same way (the two streams' TRSM and GEMM may run at once)
+        if (const char *v = std::getenv("IEMIC_LU_LOOKAHEAD")) p->lookahead = std::atoi(v) != 0;
+        if (p->lookahead) {
+            size_t ws2 = 0;
+            HIPCHK(hipStreamCreateWithFlags(&p->ahead.s2, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&p->ahead.ev_panel, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&p->ahead.ev_rest, hipEventDisableTiming));
+            BLASCHK(rocblas_create_handle(&p->ahead.h2));
+            BLASCHK(rocblas_set_stream(p->ahead.h2, p->ahead.s2));
+            BLASCHK(rocblas_start_device_memory_size_query(p->ahead.h2));
+            dense_getrf_size_query(p->ahead.h2, p->nmax);
+            BLASCHK(rocblas_stop_device_memory_size_query(p->ahead.h2, &ws2));
+            BLASCHK(rocblas_set_device_memory_size(p->ahead.h2, ws2));
+        }
     }
     return IEMIC_OK;
 }
@@ -578,6 +594,7 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
     if (!p->d_w) HIPCHK(hipMalloc(&p->d_w, std::max<size_t>((size_t)p->nmax * p->nmax, 1) * sizeof(double)));
     const RowGeom G{p->d_loc, c->cfg.n, c->cfg.m, c->cfg.l, c->cfg.periodic};
     BLASCHK(rocblas_set_pointer_mode(p->blas, rocblas_pointer_mode_host));
+    if (p->ahead.h2) BLASCHK(rocblas_set_pointer_mode(p->ahead.h2, rocblas_pointer_mode_host));
     const double sign = (double)c->cfg.int_sign;
     const int nbc = nblk(c->sc.ncell, 256);
     hipLaunchKernelGGL(k_row_scale, dim3(nbc), dim3(256), 0, c->stream, c->sc, G, c->d_J, c->d_coef, sign, p->d_pin,
@@ -609,7 +626,8 @@ static int rowlu_factor(iemic_ctx *c, RowFactors<T> &F)
         // Minv_j: solve D W = I, then W replaces the factors
         T *W = reinterpret_cast<T *>(p->d_w);
         if (p->dense_lu) {   // the inverse replaces the factors in D
-            int rc = dense_getrf<T>(p->blas, c->stream, nr, D, p->d_ipiv, p->d_info + j, p->d_luw, p->d_rowperm);
+            int rc = dense_getrf<T>(p->blas, c->stream, nr, D, p->d_ipiv, p->d_info + j, p->d_luw, p->d_rowperm,
+                                    p->lookahead ? &p->ahead : nullptr);
             if (!rc) rc = dense_inverse<T>(p->blas, c->stream, nr, D, p->d_rowperm, W);
             if (rc) return rc;
             continue;

@@ -160,6 +160,43 @@ def test_row_lu_is_exact_for_the_factored_matrix(name, lu):
 
 
 @pytest.mark.parametrize("name", ["gateway16", "global96"])
+def test_row_lu_with_look_ahead(name):
+    """The tournament-pivoted LU with its panels factored one outer block
+    ahead on a second stream (the row interchanges outside a panel replayed
+    later from the recorded move lists): the exactness of the plain build
+    (J M^{-1} v = v off the replaced rows, to the same backward error), and
+    the same M^{-1} v up to the rounding of the differently split trailing
+    products, which the explicit inverses of these ill-conditioned blocks
+    amplify (gateway16: 5e-6 of max |z|; a misplaced interchange would break
+    the exactness instead)."""
+    case = _case(name)
+    outs = {}
+    for mode in ("0", "1"):
+        os.environ["IEMIC_LU_LOOKAHEAD"] = mode
+        try:
+            oc = gpu_ocean(case)
+            oc.computeJacobian(case.x)
+            oc.buildPreconditioner()
+        finally:
+            del os.environ["IEMIC_LU_LOOKAHEAD"]
+        J = _csr(oc)
+        v = J @ np.random.default_rng(3).normal(size=oc.ndim)
+        outs[mode] = (oc.applyPrecon(v), oc.preconditionerInfo().pins)
+        oc.close()
+    z0, pins = outs["0"]
+    z1, _ = outs["1"]
+    d = np.abs(z1 - z0).max() / np.abs(z0).max()
+    r0 = np.abs(J @ z0 - v) / (abs(J) @ np.abs(z0) + np.abs(v))
+    r1 = np.abs(J @ z1 - v) / (abs(J) @ np.abs(z1) + np.abs(v))
+    e0, e1 = np.sort(r0)[-pins - 2], np.sort(r1)[-pins - 2]
+    print(f"{name}: look-ahead against one stream {d:.2e}, rows off {(r1 > 1e-6).sum()} (pins {pins}), "
+          f"backward error off the replaced rows {e1:.1e} (one stream {e0:.1e})")
+    assert (r1 > 1e-6).sum() <= pins + 1
+    assert e1 <= 10 * e0 + 1e-12
+    assert d <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["gateway16", "global96"])
 def test_row_lu_with_rocsolver_lu(name):
     """The row blocks factored by rocSOLVER getrf (partial pivoting) and
     inverted by getrs instead of the library's tournament-pivoted LU: the
