@@ -53,6 +53,7 @@
 #include <climits>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 
 #include "ctx.hpp"
 #include "dense_lu.hpp"
@@ -475,6 +476,63 @@ __global__ void __launch_bounds__(256) k_calu_update(T *__restrict__ A, int lda,
     }
 }
 
+typedef double lu_d4 __attribute__((ext_vector_type(4)));
+
+// Double precision: the rank-w update of k_calu_update on the matrix cores
+// (v_mfma_f64_16x16x4f64, lane l: A[l % 16][l / 16], B[l / 16][l % 16],
+// value v: D[l / 16 + 4 v][l % 16]).  Workgroup (x, y): rows [r0, r0 + 64)
+// below the sub-panel, columns [jc, jc + 64) of [k + w, c1), one 16-column
+// chunk per wave.  The product is taken transposed, (L21 U12)^T = U12^T
+// L21^T: M = column, N = row, K = sub-panel column c; at step m lane l holds
+// U12[c][j + l % 16] and L21[r + l % 16][c] (c = 4 m + l / 16: 16 lanes read
+// 128 contiguous bytes of one column of L21), and value v of row tile t is
+// the update of row r0 + 16 t + l % 16, column j + l / 16 + 4 v.
+__global__ void __launch_bounds__(256) k_calu_update_mfma(double *__restrict__ A, int lda, int n, int k, int w,
+                                                          int c1)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kk = lane >> 4, ii = lane & 15;
+    const int r0 = k + w + blockIdx.x * 64;
+    const int jc = k + w + blockIdx.y * 64 + 16 * wave;
+    if (jc >= c1) return;   // uniform per wave
+    const int j = jc + ii;
+    lu_d4 acc[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) acc[rt] = lu_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < CA_W / 4; m++) {
+        const int c = 4 * m + kk;
+        const bool cv = c < w;
+        const double h = (cv && j < c1) ? A[(k + c) + (size_t)j * lda] : 0.0;
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) {
+            const int r = r0 + 16 * rt + ii;
+            const double x = (cv && r < n) ? A[r + (size_t)(k + c) * lda] : 0.0;
+            acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(h, x, acc[rt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) {
+        const int r = r0 + 16 * rt + ii;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int jj = jc + kk + 4 * v;
+            if (r < n && jj < c1) A[r + (size_t)jj * lda] -= acc[rt][v];
+        }
+    }
+}
+
+// the block update on the matrix cores (double only; IEMIC_LU_MFMA=0 keeps
+// k_calu_update for comparison runs)
+static bool lu_mfma()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("IEMIC_LU_MFMA");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return on;
+}
+
 static rocblas_status trsm_llnu(rocblas_handle h, int m, int n, const double *L, int lda, double *B, int ldb)
 {
     const double one = 1.0;
@@ -615,6 +673,14 @@ int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *in
             const int strips = (c1 - (k + w) + CA_STRIP - 1) / CA_STRIP;
             hipLaunchKernelGGL(k_calu_l21<T>, dim3(nrowblk + strips), dim3(256), 0, s, A, n, n, k, w, c1, lu11,
                                nrowblk);
+            if constexpr (std::is_same<T, double>::value) {
+                if (lu_mfma()) {
+                    if (strips > 0 && below > 0)
+                        hipLaunchKernelGGL(k_calu_update_mfma, dim3((below + 63) / 64, (c1 - (k + w) + 63) / 64),
+                                           dim3(256), 0, s, A, n, n, k, w, c1);
+                    continue;
+                }
+            }
             if (strips > 0 && below > 0) {
                 const int chunk = 256;
                 dim3 grid(strips, (below + chunk - 1) / chunk);
