@@ -304,6 +304,159 @@ __global__ void __launch_bounds__(G)
     if (lane == 0) moves[0] = __popcll(bm);
 }
 
+// The same selection with two rows per thread, rows t and t + G/2 of the
+// group on G/2 threads: a 512-row group runs one wave per SIMD instead of
+// two, and each broadcast of the pivot row (the readlanes) serves both rows.
+// On ties between a thread's two rows the first is taken.
+template <typename T, int G, bool FINAL>
+__global__ void __launch_bounds__(G / 2)
+    k_calu_select2(const T *__restrict__ A, int lda, int k, int w, const int *__restrict__ rows_in, int cnt,
+                   int *__restrict__ rows_out, T *__restrict__ lu11, int *__restrict__ ipiv, int *__restrict__ moves,
+                   int *__restrict__ info)
+{
+    constexpr int NT = G / 2, NW = NT / 64;
+    __shared__ T wrow[2][NW][CA_W];
+    __shared__ unsigned long long wk[2][NW];
+    __shared__ int wt[2][NW];
+    __shared__ T s_u[FINAL ? CA_W : 1][CA_W + 1];
+    __shared__ T mult[FINAL ? G : 1][CA_W + 1];
+    __shared__ int s_selt[CA_W];
+    __shared__ int s_sel[CA_W];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int idx0 = blockIdx.x * G + t, idx1 = idx0 + NT;
+    const int row0 = idx0 < cnt ? (rows_in ? rows_in[idx0] : k + idx0) : -1;
+    const int row1 = idx1 < cnt ? (rows_in ? rows_in[idx1] : k + idx1) : -1;
+    bool live0 = row0 >= 0, live1 = row1 >= 0;
+    T a0[CA_W], a1[CA_W];
+#pragma unroll
+    for (int c = 0; c < CA_W; c++) {
+        a0[c] = (live0 && c < w) ? A[row0 + (size_t)(k + c) * lda] : (T)0;
+        a1[c] = (live1 && c < w) ? A[row1 + (size_t)(k + c) * lda] : (T)0;
+    }
+    if (t < CA_W) s_sel[t] = -1;
+    __syncthreads();
+    bool done = false;   // uniform: no rows left in this group
+#pragma unroll
+    for (int cb = 0; cb < CA_W; cb += 8) {
+        if (cb >= w || done) break;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = cb + j;   // a compile-time constant in each unrolled step
+            if (c >= w || done) break;
+            const T ac0 = a0[c], ac1 = a1[c];
+            const bool take1 = live1 && (!live0 || fabs(ac1) > fabs(ac0));
+            const int par = c & 1;
+            unsigned long long key;
+            const int fl = wave_argmax_abs(take1 ? ac1 : ac0, live0 || live1, &key);
+            if (lane == fl) {
+#pragma unroll
+                for (int cc = 0; cc < CA_W; cc++)
+                    if (cc >= c) wrow[par][wave][cc] = take1 ? a1[cc] : a0[cc];
+                wt[par][wave] = wave * 64 + lane + (take1 ? NT : 0);
+            }
+            if (lane == 0) wk[par][wave] = key;
+            __syncthreads();
+            unsigned long long bk = wk[par][0];
+            int bw = 0;
+#pragma unroll
+            for (int q = 1; q < NW; q++)
+                if (wk[par][q] > bk) {
+                    bk = wk[par][q];
+                    bw = q;
+                }
+            if (bk == 0) {   // uniform
+                done = true;
+                break;
+            }
+            const int pt = wt[par][bw];   // row of the group: thread pt % NT, half pt / NT
+            const T myv = wrow[par][bw][lane & (CA_W - 1)];
+            const T piv = lane_value(myv, c);
+            if (t == pt) {
+                s_sel[c] = row0;
+                live0 = false;
+            } else if (t + NT == pt) {
+                s_sel[c] = row1;
+                live1 = false;
+            }
+            if (FINAL) {
+                if (wave == 0 && lane < CA_W) s_u[c][lane] = lane >= c ? myv : (T)0;
+                if (t == 0) {
+                    s_selt[c] = pt;
+                    if (piv == (T)0 && *info == 0) *info = k + c + 1;
+                }
+            }
+            const T l0 = (live0 && piv != (T)0) ? ac0 / piv : (T)0;
+            const T l1 = (live1 && piv != (T)0) ? ac1 / piv : (T)0;
+            if (FINAL) {
+                if (live0) mult[t][c] = l0;
+                if (live1) mult[t + NT][c] = l1;
+            }
+#pragma unroll
+            for (int cc = 0; cc < CA_W; cc++)
+                if (cc > c) {
+                    const T pv = lane_value(myv, cc);
+                    a0[cc] = fma(-l0, pv, a0[cc]);
+                    a1[cc] = fma(-l1, pv, a1[cc]);
+                }
+        }
+    }
+    __syncthreads();
+    if (FINAL && t < w) {
+        // row t of L11 (the multipliers of the t-th chosen row against the
+        // pivots before it) and of U11
+        const int tt = s_selt[t];
+#pragma unroll
+        for (int p = 0; p < CA_W; p++)
+            if (p < w) lu11[t + p * CA_W] = p < t ? mult[tt][p] : s_u[t][p];
+    }
+    __syncthreads();
+    if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
+    if (!FINAL || wave != 0) return;
+    // LAPACK interchange sequence and the net row moves.  Lane e holds entry
+    // e of the map "position key[e] now holds original row val[e]" over the
+    // positions touched so far (at most 2 w <= 64).
+    int key = -1, val = -1, size = 0;
+    for (int c = 0; c < w; c++) {
+        const int s = s_sel[c];
+        const unsigned long long bv = __ballot(lane < size && val == s);
+        const int p = bv ? __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)bv) - 1) : s;
+        const unsigned long long bk = __ballot(lane < size && key == k + c);
+        const int cur = bk ? __builtin_amdgcn_readlane(val, __ffsll((unsigned long long)bk) - 1) : k + c;
+        if (lane == 0) ipiv[k + c] = p + 1;
+        // position k + c receives s, position p receives cur
+        if (bk) {
+            if (lane == __ffsll((unsigned long long)bk) - 1) val = s;
+        } else {
+            if (lane == size) {
+                key = k + c;
+                val = s;
+            }
+            size++;
+        }
+        if (p != k + c) {
+            const unsigned long long bp = __ballot(lane < size && key == p);
+            if (bp) {
+                if (lane == __ffsll((unsigned long long)bp) - 1) val = cur;
+            } else {
+                if (lane == size) {
+                    key = p;
+                    val = cur;
+                }
+                size++;
+            }
+        }
+    }
+    const bool mv = lane < size && key != val;
+    const unsigned long long bm = __ballot(mv);
+    if (mv) {
+        const int slot = __popcll(bm & ((1ull << lane) - 1));
+        moves[1 + slot] = key;              // destination
+        moves[1 + CA_MOVES + slot] = val;   // source
+    }
+    if (lane == 0) moves[0] = __popcll(bm);
+}
+
+
 // The row moves of one sub-panel on columns [c0, c1) and, as one more
 // column, on the row permutation perm (when given): all sources of a column
 // are read before any destination is written (one column per 64 threads,
@@ -567,10 +720,29 @@ static int select_group(int cnt)
     return CA_G;
 }
 
+// two rows per thread for the 512-row groups (IEMIC_LU_SELECT2=0: one)
+static bool select_two_rows()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("IEMIC_LU_SELECT2");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return on;
+}
+
 template <typename T, int G>
 static void launch_select_g(int groups, hipStream_t s, const T *A, int lda, int k, int w, const int *in, int cnt,
                             int *out, int fin, T *lu11, int *ipiv, int *moves, int *info)
 {
+    if (G == CA_G && select_two_rows()) {
+        if (fin)
+            hipLaunchKernelGGL((k_calu_select2<T, G, true>), dim3(groups), dim3(G / 2), 0, s, A, lda, k, w, in, cnt,
+                               out, lu11, ipiv, moves, info);
+        else
+            hipLaunchKernelGGL((k_calu_select2<T, G, false>), dim3(groups), dim3(G / 2), 0, s, A, lda, k, w, in, cnt,
+                               out, lu11, ipiv, moves, info);
+        return;
+    }
     if (fin)
         hipLaunchKernelGGL((k_calu_select<T, G, true>), dim3(groups), dim3(G), 0, s, A, lda, k, w, in, cnt, out, lu11,
                            ipiv, moves, info);
