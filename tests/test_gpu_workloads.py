@@ -10,7 +10,8 @@ on the host from the exported Jacobian:
   host threads and streams and one after the other;
 * the 96x38x12 seeded basin: pseudo-arclength continuation in COMB from
   rest with the reference's continuation defaults (Continuation.H:1330-1385),
-  ten steps;
+  ten steps, and the limit point near COMB = 0.118 that keeps it from
+  COMB = 1 in ten steps;
 * the preconditioner with FP32 factors (precision = 1) on the closed basins
   where the FP32 row-block LU is accurate enough to converge: the FP64 bar,
   and its apply against the FP64 factors' at an FP32-level tolerance.
@@ -160,6 +161,54 @@ def test_continuation_on_the_96x38x12_basin():
     F = oc.computeRHS(x)
     print(f"|F| at the last point {np.linalg.norm(F):.2e} (COMB {ph[-1]:.4f}, |x| {nh[-1]:.3e})")
     assert np.linalg.norm(F) <= 1e-3
+    oc.close()
+
+
+def test_the_96x38x12_branch_folds_before_comb_1():
+    """Why ten steps do not reach COMB = 1 from rest on this basin: the branch
+    has a limit point near COMB = 0.118.  Continued for 60 steps (the
+    reference's defaults), COMB rises to an interior maximum and then falls
+    while every step's corrector converges; and at the last step's COMB two
+    distinct steady states exist: the one Newton refines from the step-60
+    state past the fold, and the one it finds from the step-51 state before
+    it, each much closer to its own root (its next Newton update) than to
+    the other.  (A 160-step run continues through further limit points at
+    COMB = 0.1164, 0.1209, 0.1412 and 0.1251.)"""
+    def run(steps):
+        oc, _ = _global(96, 38, 12)
+        oc.setPar("COMB", 0.0)
+        oc.setPreconditionerReuse(8)
+        x, ph, nh, info = oc.continuation(np.zeros(oc.ndim), oc.continuation_params(par=19, target=1.0,
+                                                                                    max_steps=steps))
+        return oc, x, ph, info
+    oc_b, x_b, ph_b, info_b = run(60)
+    oc_b.close()
+    oc, x_a, ph_a, info_a = run(51)
+    assert info_a.steps == 51 and info_b.steps == 60
+    assert np.array_equal(ph_a, ph_b[:52])
+    k = int(np.argmax(ph_b))
+    print("COMB around the fold", np.array2string(ph_b[k - 3:], precision=5))
+    assert 51 < k < 60 and ph_b[-1] < ph_b[k] - 5e-4
+    assert np.all(np.diff(ph_b[:k + 1]) > 0) and np.all(np.diff(ph_b[k + 2:]) < 0)
+    c = float(ph_b[-1])
+    oc.setPar("COMB", c)
+    x_u, its_u, fu = oc.newton(x_b, tol=1e-10, step_tol=1e-14, maxit=30)
+    x_l, its_l, fl = oc.newton(x_a, tol=1e-10, step_tol=1e-14, maxit=30)
+    gap = np.linalg.norm(x_l - x_u)
+    # distance of each state to its root: the next Newton update J^{-1} F
+    steps = []
+    for x in (x_u, x_l):
+        F = oc.computeRHS(x)
+        oc.computeJacobian(x)
+        dx, _ = oc.solve(F, oc.solver_params(tol=1e-6, restart=50, maxit=200), raise_on_noconv=False)
+        steps.append(np.linalg.norm(dx))
+    print(f"COMB {c:.5f}: |F| {fu:.1e} on the branch past the fold ({its_u} Newton iterations from the step-60 "
+          f"state), {fl:.1e} before it ({its_l} from the step-51 state, COMB {ph_a[-1]:.5f}); the two states differ "
+          f"by {gap:.3e} ({gap / np.linalg.norm(x_u):.4f} of |x|), their next Newton updates are "
+          f"{steps[0]:.1e} and {steps[1]:.1e}")
+    assert fu <= 1e-9 and fl <= 1e-9
+    assert max(steps) <= 1e-6 * gap
+    assert gap >= 1e-3 * np.linalg.norm(x_u)
     oc.close()
 
 
