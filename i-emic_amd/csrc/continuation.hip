@@ -8,6 +8,8 @@
 // one batched sweep.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -290,6 +292,9 @@ int continuation_dev(iemic_ctx *c, const iemic_solver_params *sp, const iemic_co
         }
         steps++;
         record(steps);
+        if (sp->verbose)
+            std::fprintf(stderr, "continuation: step %d, parameter %.8g, |x| %.6g, ds %.4g, %d Newton iterations\n",
+                         steps, p, w.norm(d_x), ds, it);
         // secant tangent: (x - x0, p - p0) / ds
         hipLaunchKernelGGL(k_diff_scale, w.g, w.b, 0, c->stream, xdot, d_x, x0, 1.0 / ds, n);
         pdot = (p - p0) / ds;
