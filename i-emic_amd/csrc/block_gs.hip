@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <string>
 
@@ -349,5 +350,51 @@ extern "C" int iemic_block_update_dev(int n, int a, int b, double alpha, const d
     hipError_t e = ts_update(nullptr, n, a, b, alpha, d_A, (size_t)n, d_H, a, beta, d_C, (size_t)n);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return set_error(IEMIC_EHIP, std::string("block update: ") + hipGetErrorString(e));
+    return IEMIC_OK;
+}
+
+namespace iemic {
+
+// y = a x + b y; with a = 0, x is not read, with b = 0, y is not read (so
+// a = b = 0 fills y with zeros whatever it held)
+__global__ void k_vec_axpby(long long n, double a, const double *x, double b, double *y)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = (a == 0.0 ? 0.0 : a * x[i]) + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+}  // namespace iemic
+
+// Level-1 operations on device vectors for host-driven iterations (the
+// Vector of the GMRES template contract, include/iemic_gmres.hpp): the null
+// stream, and the call returns when the result is in place.
+extern "C" int iemic_vec_axpby_dev(long long n, double a, const double *d_x, double b, double *d_y)
+{
+    using namespace iemic;
+    if (n < 0) return set_error(IEMIC_EINVAL, "vector update: negative length");
+    if (n == 0) return IEMIC_OK;
+    if ((a != 0.0 && !d_x) || !d_y) return set_error(IEMIC_EINVAL, "vector update: null operand");
+    const long long blocks = std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_vec_axpby, dim3((unsigned)blocks), dim3(256), 0, nullptr, n, a, d_x, b, d_y);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return set_error(IEMIC_EHIP, std::string("vector update: ") + hipGetErrorString(e));
+    return IEMIC_OK;
+}
+
+extern "C" int iemic_vec_dot_dev(long long n, const double *d_x, const double *d_y, double *out)
+{
+    using namespace iemic;
+    if (!out) return set_error(IEMIC_EINVAL, "vector dot: null result");
+    *out = 0.0;
+    if (n <= 0) return n < 0 ? set_error(IEMIC_EINVAL, "vector dot: negative length") : IEMIC_OK;
+    if (n > INT_MAX) return set_error(IEMIC_EINVAL, "vector dot: length beyond 2^31 - 1");
+    if (!d_x || !d_y) return set_error(IEMIC_EINVAL, "vector dot: null operand");
+    double *work = nullptr;
+    HIPCHK(hipMalloc(&work, (ts_gram_work((int)n, 1, 1) + 1) * sizeof(double)));
+    hipError_t e = ts_gram(nullptr, (int)n, 1, 1, d_x, (size_t)n, d_y, (size_t)n, work, 1, work + 1);
+    if (e == hipSuccess) e = hipMemcpy(out, work, sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(work);
+    if (e != hipSuccess) return set_error(IEMIC_EHIP, std::string("vector dot: ") + hipGetErrorString(e));
     return IEMIC_OK;
 }

@@ -399,6 +399,14 @@ int iemic_read_state(const char *path, double *x, int64_t n, double *par, int *f
 int iemic_write_state(const char *path, const double *x, int64_t n, const double *par);
 
 /* ---- device memory helpers for callers without their own allocator ----- */
+/* y = a x + b y (a = 0: x is not read; b = 0: y is not read) and
+ * *out = x . y on device vectors
+ * of length n; the null stream, and the call returns when done.  The
+ * level-1 operations of the GMRES template contract's Vector
+ * (GMRESSolverDecl.H:11-14: update, norm; GMRESSolver.H:178-186: dot,
+ * scale), wrapped for C++ in include/iemic_gmres.hpp. */
+int iemic_vec_axpby_dev(long long n, double a, const double *d_x, double b, double *d_y);
+int iemic_vec_dot_dev(long long n, const double *d_x, const double *d_y, double *out);
 int iemic_dev_alloc(void **ptr, long long bytes);
 int iemic_dev_free(void *ptr);
 int iemic_memcpy_h2d(void *dst, const void *src, long long bytes);
