@@ -136,6 +136,8 @@ def load():
         "iemic_solve_block_transpose_dev": ([vp, vp, C.c_int, vp, vp, vp], C.c_int),
         "iemic_block_gram_dev": ([C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
         "iemic_block_update_dev": ([C.c_int, C.c_int, C.c_int, C.c_double, vp, vp, C.c_double, vp], C.c_int),
+        "iemic_vec_axpby_dev": ([C.c_longlong, C.c_double, vp, C.c_double, vp], C.c_int),
+        "iemic_vec_dot_dev": ([C.c_longlong, vp, vp, dp], C.c_int),
         "iemic_apply_precon_multi_dev": ([vp, C.c_int, vp, vp], C.c_int),
         "iemic_newton_dev": ([vp, vp, C.c_double, C.c_double, C.c_int, vp, ip, dp], C.c_int),
         "iemic_continuation_defaults": ([vp], None),
