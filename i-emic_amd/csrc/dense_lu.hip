@@ -763,6 +763,33 @@ static void launch_select(int g, int groups, hipStream_t s, const T *A, int lda,
     }
 }
 
+constexpr int INV_NB = 1024;  // block rows of dense_inverse's GEMM-based solve with U (default; 256: +5 %, 512: +1.5 % build time at 180x80x24)
+
+// block rows of the solve with U: INV_NB, or IEMIC_INV_NB (256, 512 or
+// 1024) for comparison runs
+static int inv_block()
+{
+    static const int nb = [] {
+        const char *v = std::getenv("IEMIC_INV_NB");
+        const int b = v ? std::atoi(v) : INV_NB;
+        return (b == 256 || b == 512 || b == 1024) ? b : INV_NB;
+    }();
+    return nb;
+}
+
+static rocblas_status gemm_nn(rocblas_handle h, int m, int n, int k, double alpha, const double *A, int lda,
+                              const double *B, int ldb, double beta, double *C, int ldc)
+{
+    return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta, C,
+                         ldc);
+}
+static rocblas_status gemm_nn(rocblas_handle h, int m, int n, int k, float alpha, const float *A, int lda,
+                              const float *B, int ldb, float beta, float *C, int ldc)
+{
+    return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta, C,
+                         ldc);
+}
+
 // Inside rocblas_start/stop_device_memory_size_query: the rocBLAS calls of
 // dense_getrf at their largest shapes for an n x n block, so that the
 // handle's workspace covers them.
@@ -778,6 +805,16 @@ void dense_getrf_size_query(rocblas_handle h, int n)
             (void)gemm_nn_sub(h, n - nb, n - nb, nb, dd, n, dd, n, dd, n);
             (void)trsm_llnu(h, nb, n - nb, df, n, df, n);
             (void)gemm_nn_sub(h, n - nb, n - nb, nb, df, n, df, n, df, n);
+        }
+    // the blocked solve of dense_inverse: its GEMMs and the diagonal-block solves
+    for (int nb = 256; nb <= 1024; nb *= 2)
+        if (n > nb) {
+            (void)gemm_nn(h, nb, n, n - nb, -1.0, dd, n, dd, n, 1.0, dd, n);
+            (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                rocblas_diagonal_non_unit, nb, n, &one, dd, n, dd, n);
+            (void)gemm_nn(h, nb, n, n - nb, -1.0f, df, n, df, n, 1.0f, df, n);
+            (void)rocblas_strsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                rocblas_diagonal_non_unit, nb, n, &onef, df, n, df, n);
         }
     // the trtri and the n x n solve of dense_inverse
     (void)rocblas_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, dd, n, dd, n);
@@ -941,11 +978,43 @@ static rocblas_status trtri_lower_unit(rocblas_handle h, int n, const float *L, 
     return rocblas_strtri(h, rocblas_fill_lower, rocblas_diagonal_unit, n, L, n, W, n);
 }
 
+static rocblas_status trsm_lunn_block(rocblas_handle h, int m, int n, const double *U, int ldu, double *B, int ldb)
+{
+    const double one = 1.0;
+    return rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                         m, n, &one, U, ldu, B, ldb);
+}
+static rocblas_status trsm_lunn_block(rocblas_handle h, int m, int n, const float *U, int ldu, float *B, int ldb)
+{
+    const float one = 1.0f;
+    return rocblas_strsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                         m, n, &one, U, ldu, B, ldb);
+}
+
+// the blocked solve with U (IEMIC_INV_BLOCKED=0: one rocBLAS TRSM)
+static bool inv_blocked()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("IEMIC_INV_BLOCKED");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return on;
+}
+
 // A^{-1} from the factors of dense_getrf, written over them: W = L^{-1}
-// (trtri, n^3/3 flops), W = U^{-1} W (triangular solve, n^3), then the
-// columns of W scattered by the row permutation (A^{-1} = U^{-1} L^{-1} P^T).
-// Against two triangular solves with the permuted identity (2 n^3) this
-// saves a third of the inverse's flops.  W is n x n workspace.
+// (trtri, n^3/3 flops), X = U^{-1} W (n^3), then the columns of X scattered
+// by the row permutation (A^{-1} = U^{-1} L^{-1} P^T).  Against two
+// triangular solves with the permuted identity (2 n^3) this saves a third of
+// the inverse's flops.  W is n x n workspace.
+//
+// The solve with U runs by block rows of INV_NB from the bottom, in place
+// on W:  W_i <- W_i - U_i,>i X_>i  (one GEMM of INV_NB x n x the rows
+// below), then  X_i = U_ii^{-1} W_i  by a triangular solve with the
+// diagonal block alone.  The GEMMs carry all but INV_NB / n of the flops:
+// rocBLAS's TRSM of a whole 8192 x 8192 block ran at 18 TFLOP/s on one
+// MI355X against 71 for its GEMM.  (Multiplying by explicit inverses of the
+// diagonal blocks instead was as fast but lost two digits on the
+// ill-conditioned global basins.)
 template <typename T>
 int dense_inverse(rocblas_handle h, hipStream_t s, int n, T *LU, const int *perm, T *W)
 {
@@ -955,7 +1024,17 @@ int dense_inverse(rocblas_handle h, hipStream_t s, int n, T *LU, const int *perm
     LU_BLASCHK(trtri_lower_unit(h, n, LU, W));
     hipLaunchKernelGGL(k_lu_unit_lower<T>, dim3(grid), dim3(256), 0, s, W, n);
     HIPCHK(hipGetLastError());
-    LU_BLASCHK(trsm_lunn(h, n, LU, W));
+    const int NB = inv_block();
+    if (!inv_blocked() || n <= NB) {
+        LU_BLASCHK(trsm_lunn(h, n, LU, W));
+    } else {
+        for (int i0 = ((n - 1) / NB) * NB; i0 >= 0; i0 -= NB) {
+            const int b = std::min(NB, n - i0), r = i0 + b;
+            if (r < n)
+                LU_BLASCHK(gemm_nn(h, b, n, n - r, (T)-1, LU + i0 + (size_t)r * n, n, W + r, n, (T)1, W + i0, n));
+            LU_BLASCHK(trsm_lunn_block(h, b, n, LU + i0 + (size_t)i0 * n, n, W + i0, n));
+        }
+    }
     hipLaunchKernelGGL(k_lu_scatter_cols<T>, dim3(grid), dim3(256), 0, s, W, LU, n, perm);
     HIPCHK(hipGetLastError());
     return IEMIC_OK;
