@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "ctx.hpp"
@@ -382,6 +383,17 @@ extern "C" int iemic_vec_axpby_dev(long long n, double a, const double *d_x, dou
     return IEMIC_OK;
 }
 
+namespace iemic {
+
+// The partials of iemic_vec_dot_dev: one buffer per device, grown on demand
+// and kept until the process ends; the mutex serialises the dots of one
+// process (each returns only when its result is on the host).
+static std::mutex g_dot_mu;
+static double *g_dot_work[64] = {};
+static size_t g_dot_cap[64] = {};
+
+}  // namespace iemic
+
 extern "C" int iemic_vec_dot_dev(long long n, const double *d_x, const double *d_y, double *out)
 {
     using namespace iemic;
@@ -390,11 +402,21 @@ extern "C" int iemic_vec_dot_dev(long long n, const double *d_x, const double *d
     if (n <= 0) return n < 0 ? set_error(IEMIC_EINVAL, "vector dot: negative length") : IEMIC_OK;
     if (n > INT_MAX) return set_error(IEMIC_EINVAL, "vector dot: length beyond 2^31 - 1");
     if (!d_x || !d_y) return set_error(IEMIC_EINVAL, "vector dot: null operand");
-    double *work = nullptr;
-    HIPCHK(hipMalloc(&work, (ts_gram_work((int)n, 1, 1) + 1) * sizeof(double)));
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_error(IEMIC_EUNSUPPORTED, "vector dot: device index out of range");
+    std::lock_guard<std::mutex> lock(g_dot_mu);
+    const size_t need = ts_gram_work((int)n, 1, 1) + 1;
+    if (g_dot_cap[dev] < need) {
+        if (g_dot_work[dev]) (void)hipFree(g_dot_work[dev]);
+        g_dot_work[dev] = nullptr;
+        g_dot_cap[dev] = 0;
+        HIPCHK(hipMalloc(&g_dot_work[dev], need * sizeof(double)));
+        g_dot_cap[dev] = need;
+    }
+    double *work = g_dot_work[dev];
     hipError_t e = ts_gram(nullptr, (int)n, 1, 1, d_x, (size_t)n, d_y, (size_t)n, work, 1, work + 1);
     if (e == hipSuccess) e = hipMemcpy(out, work, sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(work);
     if (e != hipSuccess) return set_error(IEMIC_EHIP, std::string("vector dot: ") + hipGetErrorString(e));
     return IEMIC_OK;
 }
