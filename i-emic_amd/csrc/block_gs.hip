@@ -214,7 +214,6 @@ __global__ void __launch_bounds__(256) k_tsupdate(int n, int a, int b, double al
         rv[t] = r < n;
         rowoff[t] = rv[t] ? (size_t)r : (size_t)rbase;
     }
-#pragma unroll 4
     for (int c0 = 0; c0 < a; c0 += 4) {
         const int c = c0 + kk;
         const bool cv = c < a;
