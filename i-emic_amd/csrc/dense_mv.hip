@@ -115,7 +115,11 @@ __global__ void __launch_bounds__(MM_ROWS) k_mm_partial_mfma(int m, int k, int q
             xs[cc][j] = (cc < nc && j < q) ? X[(size_t)j * ldx + cb + cc] : 0.0;
         }
         __syncthreads();
-        for (int kk = 0; kk < nc; kk += 4) {
+        // a fixed trip count (steps at or beyond nc multiply zeros), so that
+        // the loop around the convergent matrix-core builtin unrolls and
+        // the factor loads of later steps are issued early
+#pragma unroll
+        for (int kk = 0; kk < MM_KC; kk += 4) {
             const int kc = kk + lk;   // < MM_KC; rows of xs at or beyond nc are zero
             double a[4];
 #pragma unroll
