@@ -120,8 +120,13 @@ __device__ __forceinline__ float lane_value(float v, int src)
 
 // Order-preserving key of |v| over the lanes of a wave: the bits of |v| (a
 // non-negative double orders like its bit pattern) with the top bit set on
-// live lanes, 0 on the others.  Returns the first lane holding the largest
-// key (-1 when no lane is live) and the key itself.
+// live lanes, 0 on the others.  Returns the first lane whose key's high word
+// (sign-free exponent and 20 leading mantissa bits of |v|) is the largest
+// (-1 when no lane is live) and that lane's full key.  Taking the first of
+// the lanes that tie in the high word rather than the largest full key saves
+// a second wave reduction per column step; the pivot chosen is within
+// 2^-20 of the largest magnitude, which partial pivoting's bound on the
+// multipliers does not notice.
 template <typename T>
 __device__ __forceinline__ int wave_argmax_abs(T v, bool live, unsigned long long *key)
 {
@@ -136,10 +141,11 @@ __device__ __forceinline__ int wave_argmax_abs(T v, bool live, unsigned long lon
     }
     hi = live ? (hi | 0x80000000u) : 0u;
     const unsigned mh = wave_max_u32(hi);
-    const unsigned ml = sizeof(T) == 8 ? wave_max_u32(hi == mh ? lo : 0u) : 0u;
-    const unsigned long long bal = __ballot(hi == mh && lo == ml);
+    const unsigned long long bal = __ballot(hi == mh);
+    const int fl = mh ? (int)__ffsll(bal) - 1 : -1;
+    const unsigned ml = fl >= 0 ? (unsigned)__builtin_amdgcn_readlane((int)lo, fl) : 0u;
     *key = ((unsigned long long)mh << 32) | ml;
-    return mh ? (int)__ffsll(bal) - 1 : -1;
+    return fl;
 }
 
 // Gaussian elimination with partial pivoting on the rows of one group,
