@@ -391,8 +391,11 @@ __global__ void __launch_bounds__(G / 2)
                     if (piv == (T)0 && *info == 0) *info = k + c + 1;
                 }
             }
-            const T l0 = (live0 && piv != (T)0) ? ac0 / piv : (T)0;
-            const T l1 = (live1 && piv != (T)0) ? ac1 / piv : (T)0;
+            // one reciprocal for both rows (LAPACK's getf2 scales by the
+            // pivot's reciprocal too)
+            const T rinv = piv != (T)0 ? (T)1 / piv : (T)0;
+            const T l0 = live0 ? ac0 * rinv : (T)0;
+            const T l1 = live1 ? ac1 * rinv : (T)0;
             if (FINAL) {
                 if (live0) mult[t][c] = l0;
                 if (live1) mult[t + NT][c] = l1;
