@@ -94,15 +94,17 @@ size_t dense_getrf_work_bytes(int n)
 
 // wave64 maximum of unsigned 32-bit values, returned to every lane: DPP row
 // shifts by 1, 2, 4, 8 (lane 15 of each row of 16 then holds the row's
-// maximum), row broadcasts of lanes 15 and 31, and lane 63 read out.
+// maximum), row broadcasts of lanes 15 and 31, and lane 63 read out.  Lanes
+// without a source read 0 (bound_ctrl), the identity of the maximum, so the
+// compiler folds each step into one v_max_u32 with a DPP operand.
 __device__ __forceinline__ unsigned wave_max_u32(unsigned x)
 {
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x111, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x112, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xa, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xc, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, true));
     return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
 }
 
@@ -146,6 +148,55 @@ __device__ __forceinline__ int wave_argmax_abs(T v, bool live, unsigned long lon
     const unsigned ml = fl >= 0 ? (unsigned)__builtin_amdgcn_readlane((int)lo, fl) : 0u;
     *key = ((unsigned long long)mh << 32) | ml;
     return fl;
+}
+
+// The LAPACK interchange sequence of one sub-panel (ipiv[k .. k + w),
+// 1-based) and its net row moves (moves: count, destinations, sources), from
+// the chosen rows in pivot order, lane c holding the c-th (wave 0 only).
+// Lane e keeps entry e of the map "position key[e] now holds original row
+// val[e]" over the positions touched so far (at most 2 w <= 64).
+__device__ __forceinline__ void calu_interchanges(int lane, int k, int w, int sel, int *__restrict__ ipiv,
+                                                  int *__restrict__ moves)
+{
+    int key = -1, val = -1, size = 0;
+    for (int c = 0; c < w; c++) {
+        const int s = __builtin_amdgcn_readlane(sel, c);
+        const unsigned long long bv = __ballot(lane < size && val == s);
+        const int p = bv ? __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)bv) - 1) : s;
+        const unsigned long long bk = __ballot(lane < size && key == k + c);
+        const int cur = bk ? __builtin_amdgcn_readlane(val, __ffsll((unsigned long long)bk) - 1) : k + c;
+        if (lane == 0) ipiv[k + c] = p + 1;
+        // position k + c receives s, position p receives cur
+        if (bk) {
+            if (lane == __ffsll((unsigned long long)bk) - 1) val = s;
+        } else {
+            if (lane == size) {
+                key = k + c;
+                val = s;
+            }
+            size++;
+        }
+        if (p != k + c) {
+            const unsigned long long bp = __ballot(lane < size && key == p);
+            if (bp) {
+                if (lane == __ffsll((unsigned long long)bp) - 1) val = cur;
+            } else {
+                if (lane == size) {
+                    key = p;
+                    val = cur;
+                }
+                size++;
+            }
+        }
+    }
+    const bool mv = lane < size && key != val;
+    const unsigned long long bm = __ballot(mv);
+    if (mv) {
+        const int slot = __popcll(bm & ((1ull << lane) - 1));
+        moves[1 + slot] = key;              // destination
+        moves[1 + CA_MOVES + slot] = val;   // source
+    }
+    if (lane == 0) moves[0] = __popcll(bm);
 }
 
 // Gaussian elimination with partial pivoting on the rows of one group,
@@ -266,48 +317,7 @@ __global__ void __launch_bounds__(G)
     __syncthreads();
     if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
     if (!FINAL || wave != 0) return;
-    // LAPACK interchange sequence and the net row moves.  Lane e holds entry
-    // e of the map "position key[e] now holds original row val[e]" over the
-    // positions touched so far (at most 2 w <= 64).
-    int key = -1, val = -1, size = 0;
-    for (int c = 0; c < w; c++) {
-        const int s = s_sel[c];
-        const unsigned long long bv = __ballot(lane < size && val == s);
-        const int p = bv ? __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)bv) - 1) : s;
-        const unsigned long long bk = __ballot(lane < size && key == k + c);
-        const int cur = bk ? __builtin_amdgcn_readlane(val, __ffsll((unsigned long long)bk) - 1) : k + c;
-        if (lane == 0) ipiv[k + c] = p + 1;
-        // position k + c receives s, position p receives cur
-        if (bk) {
-            if (lane == __ffsll((unsigned long long)bk) - 1) val = s;
-        } else {
-            if (lane == size) {
-                key = k + c;
-                val = s;
-            }
-            size++;
-        }
-        if (p != k + c) {
-            const unsigned long long bp = __ballot(lane < size && key == p);
-            if (bp) {
-                if (lane == __ffsll((unsigned long long)bp) - 1) val = cur;
-            } else {
-                if (lane == size) {
-                    key = p;
-                    val = cur;
-                }
-                size++;
-            }
-        }
-    }
-    const bool mv = lane < size && key != val;
-    const unsigned long long bm = __ballot(mv);
-    if (mv) {
-        const int slot = __popcll(bm & ((1ull << lane) - 1));
-        moves[1 + slot] = key;              // destination
-        moves[1 + CA_MOVES + slot] = val;   // source
-    }
-    if (lane == 0) moves[0] = __popcll(bm);
+    calu_interchanges(lane, k, w, lane < CA_W ? s_sel[lane] : -1, ipiv, moves);
 }
 
 // The same selection with two rows per thread, rows t and t + G/2 of the
@@ -421,48 +431,7 @@ __global__ void __launch_bounds__(G / 2)
     __syncthreads();
     if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
     if (!FINAL || wave != 0) return;
-    // LAPACK interchange sequence and the net row moves.  Lane e holds entry
-    // e of the map "position key[e] now holds original row val[e]" over the
-    // positions touched so far (at most 2 w <= 64).
-    int key = -1, val = -1, size = 0;
-    for (int c = 0; c < w; c++) {
-        const int s = s_sel[c];
-        const unsigned long long bv = __ballot(lane < size && val == s);
-        const int p = bv ? __builtin_amdgcn_readlane(key, __ffsll((unsigned long long)bv) - 1) : s;
-        const unsigned long long bk = __ballot(lane < size && key == k + c);
-        const int cur = bk ? __builtin_amdgcn_readlane(val, __ffsll((unsigned long long)bk) - 1) : k + c;
-        if (lane == 0) ipiv[k + c] = p + 1;
-        // position k + c receives s, position p receives cur
-        if (bk) {
-            if (lane == __ffsll((unsigned long long)bk) - 1) val = s;
-        } else {
-            if (lane == size) {
-                key = k + c;
-                val = s;
-            }
-            size++;
-        }
-        if (p != k + c) {
-            const unsigned long long bp = __ballot(lane < size && key == p);
-            if (bp) {
-                if (lane == __ffsll((unsigned long long)bp) - 1) val = cur;
-            } else {
-                if (lane == size) {
-                    key = p;
-                    val = cur;
-                }
-                size++;
-            }
-        }
-    }
-    const bool mv = lane < size && key != val;
-    const unsigned long long bm = __ballot(mv);
-    if (mv) {
-        const int slot = __popcll(bm & ((1ull << lane) - 1));
-        moves[1 + slot] = key;              // destination
-        moves[1 + CA_MOVES + slot] = val;   // source
-    }
-    if (lane == 0) moves[0] = __popcll(bm);
+    calu_interchanges(lane, k, w, lane < CA_W ? s_sel[lane] : -1, ipiv, moves);
 }
 
 
