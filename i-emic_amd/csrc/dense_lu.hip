@@ -741,16 +741,16 @@ static void launch_select(int g, int groups, hipStream_t s, const T *A, int lda,
     }
 }
 
-constexpr int INV_NB = 1024;  // block rows of dense_inverse's GEMM-based solve with U (default; 256: +5 %, 512: +1.5 % build time at 180x80x24)
+constexpr int INV_NB = 2048;  // block rows of dense_inverse's GEMM-based solve with U (default; 256: +5 %, 512: +1.9 %, 1024: +0.4 % build time at 180x80x24)
 
-// block rows of the solve with U: INV_NB, or IEMIC_INV_NB (256, 512 or
-// 1024) for comparison runs
+// block rows of the solve with U: INV_NB, or IEMIC_INV_NB (256, 512, 1024
+// or 2048) for comparison runs
 static int inv_block()
 {
     static const int nb = [] {
         const char *v = std::getenv("IEMIC_INV_NB");
         const int b = v ? std::atoi(v) : INV_NB;
-        return (b == 256 || b == 512 || b == 1024) ? b : INV_NB;
+        return (b == 256 || b == 512 || b == 1024 || b == 2048) ? b : INV_NB;
     }();
     return nb;
 }
@@ -785,7 +785,7 @@ void dense_getrf_size_query(rocblas_handle h, int n)
             (void)gemm_nn_sub(h, n - nb, n - nb, nb, df, n, df, n, df, n);
         }
     // the blocked solve of dense_inverse: its GEMMs and the diagonal-block solves
-    for (int nb = 256; nb <= 1024; nb *= 2)
+    for (int nb = 256; nb <= 2048; nb *= 2)
         if (n > nb) {
             (void)gemm_nn(h, nb, n, n - nb, -1.0, dd, n, dd, n, 1.0, dd, n);
             (void)rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
