@@ -221,24 +221,19 @@ __global__ void __launch_bounds__(G)
     __shared__ T wrow[2][G / 64][CA_W];
     __shared__ unsigned long long wk[2][G / 64];
     __shared__ int wt[2][G / 64];
-    // final level only: rows of U11 as they are chosen, every row's
-    // elimination multipliers (L11 is read from them), the thread of each
-    // chosen row
-    __shared__ T s_u[FINAL ? CA_W : 1][CA_W + 1];
-    __shared__ T mult[FINAL ? G : 1][CA_W + 1];
-    __shared__ int s_selt[CA_W];
     __shared__ int s_sel[CA_W];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int idx = blockIdx.x * G + t;
     const int myrow = idx < cnt ? (rows_in ? rows_in[idx] : k + idx) : -1;
     bool live = myrow >= 0;
+    int chosen = -1;   // final level: the column step that took this row
     // The row stays in registers.  The column loop is unrolled, so column c
     // is a fixed register and each step touches only the columns after c:
     // the pivot row's live part is broadcast through scalar registers and
     // the rows updated branch-free (about 4k instructions in all; a step
     // that updated every column, unrolled, was ~20k, beyond the instruction
-    // cache).  The multipliers are kept (in LDS) at the final level only,
-    // where they form L11.
+    // cache).  At the final level each multiplier replaces the entry it
+    // eliminates, so the chosen rows end as the rows of L11 \ U11.
     T a[CA_W];
 #pragma unroll
     for (int c = 0; c < CA_W; c++) a[c] = (live && c < w) ? A[myrow + (size_t)(k + c) * lda] : (T)0;
@@ -288,31 +283,25 @@ __global__ void __launch_bounds__(G)
             if (t == pt) {
                 s_sel[c] = myrow;
                 live = false;
+                chosen = c;
             }
-            if (FINAL) {
-                if (wave == 0 && lane < CA_W) s_u[c][lane] = lane >= c ? myv : (T)0;   // myv: lanes >= c were written
-                if (t == 0) {
-                    s_selt[c] = pt;
-                    if (piv == (T)0 && *info == 0) *info = k + c + 1;
-                }
-            }
+            if (FINAL && t == 0 && piv == (T)0 && *info == 0) *info = k + c + 1;
             // branch-free: rows that are no longer live (and all rows at a
             // zero pivot) take a zero multiplier and keep their values
             const T l = (live && piv != (T)0) ? ac / piv : (T)0;
-            if (FINAL && live) mult[t][c] = l;
+            if (FINAL && live) a[c] = l;   // column c is done: keep the multiplier in its place
 #pragma unroll
             for (int cc = 0; cc < CA_W; cc++)
                 if (cc > c) a[cc] = fma(-l, lane_value(myv, cc), a[cc]);
         }
     }
-    __syncthreads();
-    if (FINAL && t < w) {
-        // row t of L11 (the multipliers of the t-th chosen row against the
-        // pivots before it) and of U11
-        const int tt = s_selt[t];
+    if (FINAL && chosen >= 0) {
+        // the row chosen at step `chosen` holds its multipliers against the
+        // pivots before it and, from there on, its row of U11 (a chosen row
+        // takes no further updates): row `chosen` of L11 \ U11
 #pragma unroll
         for (int p = 0; p < CA_W; p++)
-            if (p < w) lu11[t + p * CA_W] = p < t ? mult[tt][p] : s_u[t][p];
+            if (p < w) lu11[chosen + p * CA_W] = a[p];
     }
     __syncthreads();
     if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
@@ -334,12 +323,10 @@ __global__ void __launch_bounds__(G / 2)
     __shared__ T wrow[2][NW][CA_W];
     __shared__ unsigned long long wk[2][NW];
     __shared__ int wt[2][NW];
-    __shared__ T s_u[FINAL ? CA_W : 1][CA_W + 1];
-    __shared__ T mult[FINAL ? G : 1][CA_W + 1];
-    __shared__ int s_selt[CA_W];
     __shared__ int s_sel[CA_W];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int idx0 = blockIdx.x * G + t, idx1 = idx0 + NT;
+    int chosen0 = -1, chosen1 = -1;   // final level: the column steps that took the rows
     const int row0 = idx0 < cnt ? (rows_in ? rows_in[idx0] : k + idx0) : -1;
     const int row1 = idx1 < cnt ? (rows_in ? rows_in[idx1] : k + idx1) : -1;
     bool live0 = row0 >= 0, live1 = row1 >= 0;
@@ -390,25 +377,21 @@ __global__ void __launch_bounds__(G / 2)
             if (t == pt) {
                 s_sel[c] = row0;
                 live0 = false;
+                chosen0 = c;
             } else if (t + NT == pt) {
                 s_sel[c] = row1;
                 live1 = false;
+                chosen1 = c;
             }
-            if (FINAL) {
-                if (wave == 0 && lane < CA_W) s_u[c][lane] = lane >= c ? myv : (T)0;
-                if (t == 0) {
-                    s_selt[c] = pt;
-                    if (piv == (T)0 && *info == 0) *info = k + c + 1;
-                }
-            }
+            if (FINAL && t == 0 && piv == (T)0 && *info == 0) *info = k + c + 1;
             // one reciprocal for both rows (LAPACK's getf2 scales by the
             // pivot's reciprocal too)
             const T rinv = piv != (T)0 ? (T)1 / piv : (T)0;
             const T l0 = live0 ? ac0 * rinv : (T)0;
             const T l1 = live1 ? ac1 * rinv : (T)0;
-            if (FINAL) {
-                if (live0) mult[t][c] = l0;
-                if (live1) mult[t + NT][c] = l1;
+            if (FINAL) {   // column c is done: keep the multipliers in its place
+                if (live0) a0[c] = l0;
+                if (live1) a1[c] = l1;
             }
 #pragma unroll
             for (int cc = 0; cc < CA_W; cc++)
@@ -419,14 +402,17 @@ __global__ void __launch_bounds__(G / 2)
                 }
         }
     }
-    __syncthreads();
-    if (FINAL && t < w) {
-        // row t of L11 (the multipliers of the t-th chosen row against the
-        // pivots before it) and of U11
-        const int tt = s_selt[t];
+    if (FINAL) {   // rows of L11 \ U11, as in k_calu_select
+        if (chosen0 >= 0) {
 #pragma unroll
-        for (int p = 0; p < CA_W; p++)
-            if (p < w) lu11[t + p * CA_W] = p < t ? mult[tt][p] : s_u[t][p];
+            for (int p = 0; p < CA_W; p++)
+                if (p < w) lu11[chosen0 + p * CA_W] = a0[p];
+        }
+        if (chosen1 >= 0) {
+#pragma unroll
+            for (int p = 0; p < CA_W; p++)
+                if (p < w) lu11[chosen1 + p * CA_W] = a1[p];
+        }
     }
     __syncthreads();
     if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
@@ -690,7 +676,10 @@ static rocblas_status gemm_nn_sub(rocblas_handle h, int m, int n, int k, const f
 }
 
 // Rows per selection group for a level of cnt rows: the smallest of 64,
-// 128, 256 that holds them all, else CA_G (several groups).
+// 128, 256 that holds them all, else CA_G (several groups).  (One group of
+// 2 CA_G rows for merge levels of up to 1024 candidates, which saves the
+// third tree level of row blocks above 8192 rows, measured slower: 69.7 us
+// per launch against 33.6 + 32.0 us for the two levels it replaces.)
 static int select_group(int cnt)
 {
     for (int g = 64; g < CA_G; g *= 2)
