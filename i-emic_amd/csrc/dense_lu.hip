@@ -1,8 +1,10 @@
 // Dense LU with tournament pivoting for the row-block preconditioner
-// (rowlu.hip), same contract as LAPACK getrf: A = P L U in place, L unit
-// lower, ipiv 1-based (row i was interchanged with row ipiv[i]), info = first
-// zero pivot (1-based) or 0, so LAPACK-style solvers (rocSOLVER getrs) apply
-// the factors as is; dense_inverse forms A^{-1} from them in place.
+// (rowlu.hip), in LAPACK getrf's layout: A = P L U in place, L unit lower,
+// info = first zero pivot (1-based) or 0; the row permutation is returned as
+// perm (and, with IEMIC_LU_IPIV=1, also as LAPACK's 1-based interchanges
+// ipiv, so that LAPACK-style solvers such as rocSOLVER getrs apply the
+// factors as is); dense_inverse forms A^{-1} from the factors and perm in
+// place.
 //
 // Why not rocSOLVER getrf: its panel factorisation launches three kernels
 // per column (pivot search, scale/update, singularity check) of 2-4 us each,
@@ -208,8 +210,9 @@ __device__ __forceinline__ void calu_interchanges(int lane, int k, int w, int se
 // since a column step of a 64-row group takes about half the time of a
 // 512-row group's).  At the final level (one group) the
 // workgroup also writes the factors of the chosen rows (lu11, column-major,
-// leading dimension CA_W), the LAPACK interchanges ipiv[k .. k + w), the move
-// list that performs them, and info at the first zero pivot.
+// leading dimension CA_W), info at the first zero pivot and, when ipiv is
+// given, the LAPACK interchanges ipiv[k .. k + w) and the move list that
+// performs them.
 template <typename T, int G, bool FINAL>
 __global__ void __launch_bounds__(G)
     k_calu_select(const T *__restrict__ A, int lda, int k, int w, const int *__restrict__ rows_in, int cnt,
@@ -305,7 +308,7 @@ __global__ void __launch_bounds__(G)
     }
     __syncthreads();
     if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
-    if (!FINAL || wave != 0) return;
+    if (!FINAL || wave != 0 || !ipiv) return;
     calu_interchanges(lane, k, w, lane < CA_W ? s_sel[lane] : -1, ipiv, moves);
 }
 
@@ -416,7 +419,7 @@ __global__ void __launch_bounds__(G / 2)
     }
     __syncthreads();
     if (t < CA_W) rows_out[blockIdx.x * CA_W + t] = s_sel[t];
-    if (!FINAL || wave != 0) return;
+    if (!FINAL || wave != 0 || !ipiv) return;
     calu_interchanges(lane, k, w, lane < CA_W ? s_sel[lane] : -1, ipiv, moves);
 }
 
@@ -441,6 +444,81 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     if (act) A[moves[1 + e] + (size_t)col * lda] = v;
     if (actp) perm[moves[1 + e]] = pv;
+}
+
+// The row moves of one sub-panel worked out from its chosen rows sel[0 .. w)
+// (pivot order, all >= k) instead of the LAPACK interchange sequence: row
+// sel[c] moves to position k + c, and each row of [k, k + w) that was not
+// chosen moves to one of the positions left by the chosen rows from below
+// k + w (the j-th such row, in order, to the j-th such position).  Every
+// wave works the moves out for itself (a few ballots, LDS per wave) and
+// applies them to one column of [c0, c1), or to perm as the column c1, all
+// sources read before any destination is written; workgroup 0 also records
+// the move list (count, destinations, sources) for the look-ahead replay.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_calu_swap_sel(T *__restrict__ A, int lda, int c0, int c1, int k, int w, const int *__restrict__ sel,
+                    int *__restrict__ moves, int *__restrict__ perm)
+{
+    __shared__ int s_flag[4][CA_W];   // per wave: row k + i was chosen
+    __shared__ int s_vac[4][CA_W];    // per wave: the positions left below k + w, in order
+    const int e = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int col = c0 + blockIdx.x * 4 + wv;
+    const int s = e < w ? sel[e] : -1;
+    const bool valid = e < w && s >= k;
+    const bool inq = valid && s < k + w;
+    if (e < CA_W) s_flag[wv][e] = 0;
+    __syncthreads();
+    if (inq) s_flag[wv][s - k] = 1;
+    __syncthreads();
+    const bool outq = valid && !inq;                 // position s is left free
+    const unsigned long long below = (1ull << e) - 1;
+    const unsigned long long vm = __ballot(outq);
+    // row k + e leaves [k, k + w) (as many as positions are left, which a
+    // complete selection guarantees; the bound keeps every write in range)
+    const bool leaves = e < w && s_flag[wv][e] == 0;
+    const unsigned long long dm0 = __ballot(leaves);
+    const bool disp = leaves && __popcll(dm0 & below) < __popcll(vm);
+    const unsigned long long dm = __ballot(disp);
+    if (outq) s_vac[wv][__popcll(vm & below)] = s;
+    __syncthreads();
+    const bool mc = valid && s != k + e;
+    const int vdst = disp ? s_vac[wv][__popcll(dm & below)] : -1;
+    if (blockIdx.x == 0 && wv == 0) {
+        const unsigned long long cm = __ballot(mc);
+        const int nc = __popcll(cm);
+        if (mc) {
+            const int slot = __popcll(cm & below);
+            moves[1 + slot] = k + e;
+            moves[1 + CA_MOVES + slot] = s;
+        }
+        if (disp) {
+            const int slot = nc + __popcll(dm & below);
+            moves[1 + slot] = vdst;
+            moves[1 + CA_MOVES + slot] = k + e;
+        }
+        if (e == 0) moves[0] = nc + __popcll(dm);
+    }
+    const bool act = col < c1, actp = perm && col == c1;
+    T v1 = (T)0, v2 = (T)0;
+    int p1 = 0, p2 = 0;
+    if (act) {
+        if (mc) v1 = A[s + (size_t)col * lda];
+        if (disp) v2 = A[(k + e) + (size_t)col * lda];
+    }
+    if (actp) {
+        if (mc) p1 = perm[s];
+        if (disp) p2 = perm[k + e];
+    }
+    __syncthreads();
+    if (act) {
+        if (mc) A[(k + e) + (size_t)col * lda] = v1;
+        if (disp) A[vdst + (size_t)col * lda] = v2;
+    }
+    if (actp) {
+        if (mc) perm[k + e] = p1;
+        if (disp) perm[vdst] = p2;
+    }
 }
 
 // The moves of nsub consecutive sub-panels (lists CA_MSTRIDE apart), in
@@ -687,6 +765,18 @@ static int select_group(int cnt)
     return CA_G;
 }
 
+// LAPACK interchanges (ipiv) computed by the final selection level and the
+// row moves taken from them (IEMIC_LU_IPIV=1), instead of the moves worked
+// out from the chosen rows by the swap kernel with ipiv left unwritten
+static bool lapack_ipiv()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("IEMIC_LU_IPIV");
+        return v && std::atoi(v) != 0;
+    }();
+    return on;
+}
+
 // two rows per thread for the 512-row groups (IEMIC_LU_SELECT2=0: one)
 static bool select_two_rows()
 {
@@ -799,10 +889,13 @@ void dense_getrf_size_query(rocblas_handle h, int n)
             return set_error(IEMIC_EHIP, std::string(#expr) + ": status " + std::to_string((int)st_)); \
     } while (0)
 
-// A = P L U for the n x n column-major A (lda = n); ipiv and info on the
-// device; work holds dense_getrf_work_bytes(n) bytes of device memory.
-// When perm is given it receives the row permutation (row i of P^T A is row
-// perm[i] of A).  With la (a second stream, its own rocBLAS handle and two
+// A = P L U for the n x n column-major A (lda = n); perm, ipiv and info on
+// the device; work holds dense_getrf_work_bytes(n) bytes of device memory.
+// perm receives the row permutation (row i of P^T A is row perm[i] of A).
+// ipiv (LAPACK's interchanges) is written only with IEMIC_LU_IPIV=1; by
+// default the row moves of each sub-panel come from its chosen rows alone
+// (k_calu_swap_sel), which takes the serial interchange sequence off the
+// panel's critical path, and perm is then required.  With la (a second stream, its own rocBLAS handle and two
 // events) the panels are factored one block ahead of the trailing update;
 // every launch has finished with respect to stream s when the last one
 // enqueued on s has.
@@ -812,6 +905,7 @@ int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *in
 {
     if (n <= 0) return IEMIC_OK;
     if (!work) return set_error(IEMIC_EINVAL, "dense LU: no workspace");
+    if (!perm && !lapack_ipiv()) return set_error(IEMIC_EINVAL, "dense LU: no row permutation buffer");
     const size_t ce = cand_elems(n);
     int *cand[2] = {reinterpret_cast<int *>(work), reinterpret_cast<int *>(work) + ce};
     T *lu11 = reinterpret_cast<T *>(reinterpret_cast<char *>(work) + ((2 * ce * sizeof(int) + 255) / 256) * 256);
@@ -831,18 +925,29 @@ int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *in
             int cnt = n - k;
             const int *in = nullptr;
             int lev = 0;
+            const bool lap = lapack_ipiv();
+            const int *chosen = nullptr;   // the final level's rows, in pivot order
             for (;;) {
                 const int g = select_group(cnt), groups = (cnt + g - 1) / g;
                 const int fin = groups == 1;
                 int *out = cand[lev & 1];
-                launch_select<T>(g, groups, s, A, n, k, w, in, cnt, out, fin, lu11, ipiv, moves, info);
-                if (fin) break;
+                launch_select<T>(g, groups, s, A, n, k, w, in, cnt, out, fin, lu11, lap ? ipiv : nullptr, moves,
+                                 info);
+                if (fin) {
+                    chosen = out;
+                    break;
+                }
                 in = out;
                 cnt = groups * CA_W;
                 lev++;
             }
             const int s0 = ahead ? K : 0, s1 = ahead ? K + nb : n;
-            hipLaunchKernelGGL(k_calu_swap<T>, dim3((s1 - s0 + 1 + 3) / 4), dim3(256), 0, s, A, n, s0, s1, moves, perm);
+            if (lap)
+                hipLaunchKernelGGL(k_calu_swap<T>, dim3((s1 - s0 + 1 + 3) / 4), dim3(256), 0, s, A, n, s0, s1, moves,
+                                   perm);
+            else
+                hipLaunchKernelGGL(k_calu_swap_sel<T>, dim3((s1 - s0 + 1 + 3) / 4), dim3(256), 0, s, A, n, s0, s1, k,
+                                   w, chosen, moves, perm);
             const int below = n - (k + w);
             const int nrowblk = std::max(1, (below + 255) / 256);
             const int c1 = K + nb;
