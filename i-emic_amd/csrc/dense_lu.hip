@@ -785,14 +785,12 @@ static int select_group(int cnt)
 
 // LAPACK interchanges (ipiv) computed by the final selection level and the
 // row moves taken from them (IEMIC_LU_IPIV=1), instead of the moves worked
-// out from the chosen rows by the swap kernel with ipiv left unwritten
+// out from the chosen rows by the swap kernel with ipiv left unwritten; read
+// at every factorisation, so that one process can run both forms
 static bool lapack_ipiv()
 {
-    static const bool on = [] {
-        const char *v = std::getenv("IEMIC_LU_IPIV");
-        return v && std::atoi(v) != 0;
-    }();
-    return on;
+    const char *v = std::getenv("IEMIC_LU_IPIV");
+    return v && std::atoi(v) != 0;
 }
 
 // two rows per thread for the 512-row groups (IEMIC_LU_SELECT2=0: one)
@@ -923,7 +921,8 @@ int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *in
 {
     if (n <= 0) return IEMIC_OK;
     if (!work) return set_error(IEMIC_EINVAL, "dense LU: no workspace");
-    if (!perm && !lapack_ipiv()) return set_error(IEMIC_EINVAL, "dense LU: no row permutation buffer");
+    const bool lap = lapack_ipiv();
+    if (!perm && !lap) return set_error(IEMIC_EINVAL, "dense LU: no row permutation buffer");
     const size_t ce = cand_elems(n);
     int *cand[2] = {reinterpret_cast<int *>(work), reinterpret_cast<int *>(work) + ce};
     T *lu11 = reinterpret_cast<T *>(reinterpret_cast<char *>(work) + ((2 * ce * sizeof(int) + 255) / 256) * 256);
@@ -943,7 +942,6 @@ int dense_getrf(rocblas_handle h, hipStream_t s, int n, T *A, int *ipiv, int *in
             int cnt = n - k;
             const int *in = nullptr;
             int lev = 0;
-            const bool lap = lapack_ipiv();
             const int *chosen = nullptr;   // the final level's rows, in pivot order
             for (;;) {
                 const int g = select_group(cnt), groups = (cnt + g - 1) / g;

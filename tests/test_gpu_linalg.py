@@ -129,22 +129,26 @@ def test_row_lu_solve_fp64(name):
     assert info.iterations <= 30
 
 
-@pytest.mark.parametrize("lu", ["rocsolver", "tournament"])
+@pytest.mark.parametrize("lu", ["rocsolver", "tournament", "tournament_ipiv"])
 @pytest.mark.parametrize("name", ["natl8", "box16", "gateway16", "global96"])
 def test_row_lu_is_exact_for_the_factored_matrix(name, lu):
     """M^{-1} is an exact inverse of J except in the rows the factored matrix
     replaces (one continuity row per pressure basin, the integral row):
     J M^{-1} v = v on every other row.  Both dense LUs of the row blocks:
     rocSOLVER getrf (partial pivoting) and the library's tournament-pivoted
-    LU (dense_lu.hip)."""
+    LU (dense_lu.hip), the latter with its row moves worked out from the
+    chosen rows (the default) and from LAPACK's interchange sequence
+    (IEMIC_LU_IPIV=1)."""
     case = _case(name)
-    os.environ["IEMIC_DENSE_LU"] = "1" if lu == "tournament" else "0"
+    os.environ["IEMIC_DENSE_LU"] = "0" if lu == "rocsolver" else "1"
+    os.environ["IEMIC_LU_IPIV"] = "1" if lu == "tournament_ipiv" else "0"
     try:
         oc = gpu_ocean(case)
         oc.computeJacobian(case.x)
         oc.buildPreconditioner()
     finally:
         del os.environ["IEMIC_DENSE_LU"]
+        del os.environ["IEMIC_LU_IPIV"]
     pins = oc.preconditionerInfo().pins
     J = _csr(oc)
     v = J @ np.random.default_rng(3).normal(size=oc.ndim)
