@@ -48,6 +48,9 @@ __global__ void __launch_bounds__(MM_ROWS) k_mm_partial(int m, int k, int q, int
         if (r < m) {
             const T *a = A + (size_t)cb * lda + r;
             int cc = 0;
+            // unrolled so that the loads of several steps are in flight at
+            // once (the same sums in the same order)
+#pragma unroll 4
             for (; cc + 4 <= nc; cc += 4) {
                 const T a0 = a[(size_t)cc * lda], a1 = a[(size_t)(cc + 1) * lda];
                 const T a2 = a[(size_t)(cc + 2) * lda], a3 = a[(size_t)(cc + 3) * lda];
@@ -124,8 +127,13 @@ __global__ void __launch_bounds__(MM_ROWS) k_mm_partial_mfma(int m, int k, int q
             double a[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
+                // an unconditional load (of A[0] where out of range) and a
+                // select: loads under a branch were each waited for before
+                // the step's products, one round trip per step
                 const int r = row0 + 16 * t + lr;
-                a[t] = (kc < nc && r < m) ? A[(size_t)(cb + kc) * lda + r] : 0.0;
+                const bool in = kc < nc && r < m;
+                const double v = A[in ? (size_t)(cb + kc) * lda + r : 0];
+                a[t] = in ? v : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < CT; u++) {
