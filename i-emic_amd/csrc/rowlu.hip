@@ -278,9 +278,18 @@ __global__ void __launch_bounds__(256) k_row_x(int nprev, const int *__restrict_
     T *out = X + (size_t)c * nprev;
     for (int e = threadIdx.x; e < nprev; e += blockDim.x) {
         T acc = 0;
-        for (int q = q0; q < q1; q++) {
-            acc += (T)uval[q] * Minv[(size_t)uent[q].x * nprev + e];
+        int q = q0;
+        // four entries' loads in flight at once, summed in the same order
+        for (; q + 4 <= q1; q += 4) {
+            const int2 u0 = uent[q], u1 = uent[q + 1], u2 = uent[q + 2], u3 = uent[q + 3];
+            const T m0 = Minv[(size_t)u0.x * nprev + e], m1 = Minv[(size_t)u1.x * nprev + e];
+            const T m2 = Minv[(size_t)u2.x * nprev + e], m3 = Minv[(size_t)u3.x * nprev + e];
+            acc += (T)uval[q] * m0;
+            acc += (T)uval[q + 1] * m1;
+            acc += (T)uval[q + 2] * m2;
+            acc += (T)uval[q + 3] * m3;
         }
+        for (; q < q1; q++) acc += (T)uval[q] * Minv[(size_t)uent[q].x * nprev + e];
         out[e] = acc;
     }
 }
@@ -296,7 +305,20 @@ __global__ void __launch_bounds__(256) k_row_schur(int nrow, int nprev, const in
     T *dc = D + (size_t)c * nrow;
     for (int e = threadIdx.x; e < nrow; e += blockDim.x) {
         T acc = 0;
-        for (int q = lptr[e]; q < lptr[e + 1]; q++) acc += (T)lval[q] * xc[lent[q].x];
+        int q = lptr[e];
+        const int q1 = lptr[e + 1];
+        // four entries' index loads, then their four gathers, in flight at
+        // once (one chain of dependent loads per entry before); summed in
+        // the same order
+        for (; q + 4 <= q1; q += 4) {
+            const int2 l0 = lent[q], l1 = lent[q + 1], l2 = lent[q + 2], l3 = lent[q + 3];
+            const T x0 = xc[l0.x], x1 = xc[l1.x], x2 = xc[l2.x], x3 = xc[l3.x];
+            acc += (T)lval[q] * x0;
+            acc += (T)lval[q + 1] * x1;
+            acc += (T)lval[q + 2] * x2;
+            acc += (T)lval[q + 3] * x3;
+        }
+        for (; q < q1; q++) acc += (T)lval[q] * xc[lent[q].x];
         dc[e] -= acc;
     }
 }
